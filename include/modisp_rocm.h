@@ -1,0 +1,127 @@
+/* modisp-rocm: C ABI of the MI355X (gfx950) multimode dispersion backend.
+ *
+ * This is the drop-in boundary of the `modisp-rocm` plugin. Each entry point
+ * replaces one binding of the reference's CUDA plugin (YuanYusung/modisp
+ * 0.5.0), cited per declaration. All pointers are plain host or device
+ * pointers; no framework types cross this boundary.
+ *
+ * Conventions (as the reference's legacy C ABI, cpp/cuda/coarse_scan.cu:31-47):
+ * functions return 0 on success and a non-zero code on failure, with the
+ * message available from md_rocm_last_error() (thread-local here):
+ *   1 = device/runtime error (reference: RuntimeError "QE CUDA: ...", qe.cu:823-825)
+ *   2 = invalid argument      (reference: ValueError, qe.cu:978-995)
+ *   3 = allocation failure    (reference: MemoryError, review_groups.cu:15-17)
+ * Numerical failures never fail a call; they become flags, NaN and counts.
+ */
+#ifndef MODISP_ROCM_H
+#define MODISP_ROCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MD_ROCM_ABI_VERSION 9 /* reference python/modisp/_loader.py:4 */
+#define MD_ROCM_VERSION "0.5.0"
+
+typedef struct md_rocm_solver md_rocm_solver;
+
+/* Solver configuration; the fields mirror the reference QeSolver constructor
+ * (cpp/cuda/qe.cu:1166): accuracy, close_root_rtol, single_precision,
+ * group_velocity, group_rtol, continuation, anchor_stride; plus this
+ * backend's grid size and FP32 uncertainty band. */
+typedef struct md_rocm_config {
+  int accuracy;            /* 0 high (exhaustive grid + trough probes); others accepted */
+  double close_root_rtol;  /* close-root threshold, default 0.005 */
+  int single_precision;    /* 1: FP32 grid scan, FP64 recheck/refine; 0: FP64 grid scan */
+  int group_velocity;      /* 1: analytic FP64 group velocity */
+  double group_rtol;       /* accepted for API parity; the analytic derivative is exact */
+  int continuation;        /* anchor continuation across periods (0 = independent) */
+  int anchor_stride;       /* anchors every <= stride periods, default 4 */
+  int nc;                  /* grid samples per trace: 256, 512 or 1024 (default 512) */
+  double tau_floor;        /* FP32 |F| uncertainty band floor (default 1e-4) */
+  double tau_phase;        /* band growth per radian of vertical S phase (default 5e-5) */
+  int chunk_traces;        /* traces per device chunk (0 = automatic) */
+} md_rocm_config;
+
+/* Defaults for every field. */
+void md_rocm_default_config(md_rocm_config *cfg);
+
+/* Create a solver for up to `capacity` models of `L` rows, `nf` periods and
+ * `nmodes` reported modes. Replaces QeSolver.__init__ (cpp/cuda/qe.cu:966-1038).
+ * Returns NULL on failure (see md_rocm_last_error). */
+md_rocm_solver *md_rocm_create(int capacity, int L, int nf, int nmodes, const md_rocm_config *cfg);
+
+/* Release every resource; NULL is accepted. Replaces QeSolver.close (qe.cu:1039-1059). */
+void md_rocm_free(md_rocm_solver *s);
+
+/* Solve B <= capacity models from HOST arrays. Replaces QeSolver.run
+ * (cpp/cuda/qe.cu:1060-1118):
+ *   thickness, vp, vs, rho : [B, L] row-major float64
+ *   periods                : [nf] shared, or [B, nf] when row_periods != 0
+ *   bounds                 : [B, 2] search window (lo, hi)
+ *   wave                   : 0 Rayleigh, 1 Love
+ * Outputs (host, caller-allocated):
+ *   velocity [B, nmodes, nf] float64 (or float32 when velocity_is_f32),
+ *   flags [B, nf] uint32, levels [B, nf] int32, counts [B, nf] int32,
+ *   group / group_flags [B, nmodes, nf] (may be NULL when group velocity is off),
+ *   prediction_flags [B, nf] (may be NULL), stats[nstats] uint64 (may be NULL). */
+int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                const double *rho, int B, const double *periods, int row_periods, const double *bounds,
+                double c_tol_rel, double f_accept, int wave, int automatic, void *velocity,
+                int velocity_is_f32, uint32_t *flags, int32_t *levels, int32_t *counts, double *group,
+                uint32_t *group_flags, uint32_t *prediction_flags, uint64_t *stats, int nstats);
+
+/* As md_rocm_run, but every input and output pointer is DEVICE memory on the
+ * solver's device and the call is asynchronous on the solver's stream until
+ * md_rocm_synchronize. No counterpart in the reference (whose run() copies
+ * through pinned staging, qe.cu:1091); it exists so device-resident callers
+ * (the torch.autograd binding, benchmarks) avoid host round trips. */
+int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                       const double *rho, int B, const double *periods, int row_periods, const double *bounds,
+                       double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
+                       int32_t *counts, double *group, uint32_t *group_flags);
+
+/* Ragged batch from HOST arrays: model m has rows [row_offsets[m], row_offsets[m+1])
+ * of the packed thickness/vp/vs/rho arrays, periods [period_offsets[m],
+ * period_offsets[m+1]) and modes[m] reported modes. Outputs are packed per
+ * model: velocity/group cell = out_offsets[m] + mode * T_m + f with
+ * out_offsets[m] = sum_{i<m} modes[i] T_i, flags/levels/counts index
+ * period_offsets[m] + f. The reference requires rectangular batches (padding,
+ * python/modisp/models.py:224-270); this entry point is its ragged equivalent. */
+int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                       const double *rho, const int64_t *row_offsets, int B, const double *periods,
+                       const int64_t *period_offsets, const int32_t *modes, const double *bounds,
+                       double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
+                       int32_t *counts, double *group, uint32_t *group_flags);
+
+/* Wait for the solver's stream; returns the first asynchronous error. */
+int md_rocm_synchronize(md_rocm_solver *s);
+
+/* Kernel time of the last run in milliseconds (HIP events on the solver's
+ * stream), per stage: 0 total, 1 scan, 2 recheck, 3 trough, 4 refine, 5 group.
+ * Returns the number of stages written. */
+int md_rocm_last_timing(md_rocm_solver *s, double *ms, int n);
+
+/* Automatic search window [B, 2] (reference _cpu.bounds, cpp/cpu/solver.cpp:781-790):
+ * lower = 1/8 of the slowest Rayleigh/Scholte/Love reference speed unless
+ * auto_low == 0 (then `low`), upper = min(high or Vs_halfspace, nextafter(Vs_halfspace, 0)). */
+int md_rocm_bounds(const double *thickness, const double *vp, const double *vs, const double *rho, int B, int L,
+                   int wave, int auto_low, double low, int auto_high, double high, double *out);
+
+/* Device helpers (reference cpp/cuda/qe.cu:1140-1142, bindings.cpp:354-357). */
+int md_rocm_available(int device); /* 1 if the device exists and is gfx950-capable */
+int md_rocm_set_device(int device);
+int md_rocm_memory_info(size_t *free_bytes, size_t *total_bytes);
+int md_rocm_device_name(int device, char *buf, int len);
+
+/* Thread-local message of the last failed call. */
+const char *md_rocm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MODISP_ROCM_H */

@@ -1,0 +1,247 @@
+"""modisp-rocm: the MI355X (gfx950) backend plugin for modisp-style multimode
+surface-wave dispersion.
+
+This package is the `modisp-rocm` distribution's import package. Like the
+reference CUDA plugin (cuda-plugin/python/modisp_cuda/__init__.py:1-7) it
+exposes ``load()``, returning the native module object the reference's loader
+expects (python/modisp/_loader.py:15-28): ``ABI_VERSION``, ``VERSION``, the
+``SUPPORTS_*`` capability attributes, ``available``/``set_device``/
+``memory_info``/``device_name`` and the ``QeSolver`` engine class. The engine
+calls gfx950 HIP kernels through the C ABI in include/modisp_rocm.h.
+
+``modisp_amd.api`` adds the ``Solver`` / ``dispersion_batch`` /
+``dispersion`` front end with a ``rocm`` backend entry (auto selection
+included), and ``modisp_amd.autograd`` the torch.autograd binding.
+"""
+from __future__ import annotations
+
+import ctypes
+import sys
+
+import numpy as np
+
+from . import _native
+from ._native import BackendUnavailableError
+
+__version__ = "0.5.0"
+VERSION = "0.5.0"
+ABI_VERSION = 9
+SUPPORTS_FP32 = True
+SUPPORTS_AUTO_ACCURACY = True
+SUPPORTS_CANONICAL_LOWER = True
+SUPPORTS_ROW_PERIODS = True
+SUPPORTS_GROUP_VELOCITY = True
+SUPPORTS_CONTINUATION = False
+STAT_NAMES = ("traces", "fp64_rechecks", "trough_probes", "unresolved_traces")
+
+__all__ = ["load", "QeSolver", "available", "set_device", "memory_info", "device_name", "bounds",
+           "BackendUnavailableError", "ABI_VERSION", "VERSION"]
+
+
+def load():
+    """Return the native module object (this package) after loading the library."""
+    try:
+        _native.lib()
+    except BackendUnavailableError as exc:
+        raise ImportError(str(exc)) from exc
+    return sys.modules[__name__]
+
+
+def available(device: int = 0) -> bool:
+    """True when `device` exists and is a gfx950 (MI355X) GPU."""
+    try:
+        return bool(_native.lib().md_rocm_available(int(device)))
+    except BackendUnavailableError:
+        return False
+
+
+def set_device(device: int) -> None:
+    _native.check(_native.lib().md_rocm_set_device(int(device)))
+
+
+def memory_info():
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _native.check(_native.lib().md_rocm_memory_info(ctypes.byref(free), ctypes.byref(total)))
+    return int(free.value), int(total.value)
+
+
+def device_name(device: int = 0) -> str:
+    buf = ctypes.create_string_buffer(256)
+    _native.check(_native.lib().md_rocm_device_name(int(device), buf, 256))
+    return buf.value.decode()
+
+
+def _rows(a, name):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2:
+        raise ValueError(f"{name} must be [B, L]")
+    return a
+
+
+def bounds(thickness, vp, vs, rho, wave=0, low=None, high=None):
+    """Search window [B, 2] (reference _cpu.bounds, cpp/cpu/solver.cpp:781-790)."""
+    h, a, b, r = (_rows(x, n) for x, n in zip((thickness, vp, vs, rho), ("thickness", "vp", "vs", "rho")))
+    B, L = h.shape
+    out = np.empty((B, 2))
+    code = _native.lib().md_rocm_bounds(_native.ptr(h), _native.ptr(a), _native.ptr(b), _native.ptr(r), B, L,
+                                        int(wave), int(low is None), 0. if low is None else float(low),
+                                        int(high is None), 0. if high is None else float(high), _native.ptr(out))
+    if code:
+        raise ValueError("invalid bounds geometry")
+    return out
+
+
+class QeSolver:
+    """Persistent native engine; the constructor and run() mirror the reference
+    QeSolver (cpp/cuda/qe.cu:1166 signature, run at qe.cu:1060-1118).
+
+    ``single_precision`` keeps the reference meaning of the returned dtype
+    (float32 velocities). Extra keyword-only arguments select this backend's
+    method: ``scan`` ("float32": FP32 grid scan with FP64 recheck and FP64
+    refinement, the default; "float64": the plain FP64 grid path), ``nc``
+    samples per trace and the FP32 uncertainty band (``tau_floor``,
+    ``tau_phase``).
+    """
+
+    def __init__(self, B, L, nf, nmodes, accuracy=0, period_threshold=0., close_root_rtol=.005, shape=False,
+                 sampling_rtol=.001, secular=0, refinement=1, love_phase=1, single_precision=False,
+                 group_velocity=False, group_rtol=.001, continuation=False, anchor_stride=4, *, scan="float32",
+                 nc=512, tau_floor=None, tau_phase=None, chunk_traces=0):
+        if scan not in ("float32", "float64"):
+            raise ValueError("scan must be float32 or float64")
+        if continuation:
+            raise NotImplementedError("native anchor continuation is not available in this plugin build")
+        cfg = _native.Config()
+        L_ = _native.lib()
+        L_.md_rocm_default_config(ctypes.byref(cfg))
+        cfg.accuracy = int(accuracy)
+        cfg.close_root_rtol = float(close_root_rtol)
+        cfg.single_precision = int(scan == "float32")
+        cfg.group_velocity = int(bool(group_velocity))
+        cfg.group_rtol = float(group_rtol)
+        cfg.continuation = int(bool(continuation))
+        cfg.anchor_stride = int(anchor_stride)
+        cfg.nc = int(nc)
+        if tau_floor is not None:
+            cfg.tau_floor = float(tau_floor)
+        if tau_phase is not None:
+            cfg.tau_phase = float(tau_phase)
+        cfg.chunk_traces = int(chunk_traces)
+        self.capacity, self.L, self.nf, self.nmodes = int(B), int(L), int(nf), int(nmodes)
+        self.single_precision = bool(single_precision)
+        self.scan = scan
+        self.group_velocity = bool(group_velocity)
+        self.config = cfg
+        handle = L_.md_rocm_create(self.capacity, self.L, self.nf, self.nmodes, ctypes.byref(cfg))
+        if not handle:
+            msg = _native.last_error()
+            raise (ValueError if "invalid" in msg or "must" in msg else RuntimeError)(msg)
+        self._handle = ctypes.c_void_p(handle)
+
+    @property
+    def handle(self):
+        if self._handle is None:
+            raise RuntimeError("solver is closed")
+        return self._handle
+
+    def run(self, thickness, vp, vs, rho, periods, bounds, c_tol_rel, f_accept, wave, automatic, profile=False,
+            canonical_lower=False):
+        h, a, b, r = (_rows(x, n) for x, n in zip((thickness, vp, vs, rho), ("thickness", "vp", "vs", "rho")))
+        B, L = h.shape
+        if any(x.shape != (B, L) for x in (a, b, r)):
+            raise ValueError("model shapes differ")
+        if B > self.capacity or L != self.L:
+            raise ValueError(f"model batch [{B},{L}] exceeds solver geometry [{self.capacity},{self.L}]")
+        per = np.ascontiguousarray(periods, dtype=np.float64)
+        row = 0
+        if per.ndim == 2:
+            if per.shape[0] != B or per.shape[1] != self.nf:
+                raise ValueError("row periods must be [B, nf]")
+            row = 1
+        elif per.ndim != 1 or per.size != self.nf:
+            raise ValueError("periods must have nf entries")
+        bnd = np.ascontiguousarray(bounds, dtype=np.float64).reshape(B, 2)
+        M, T = self.nmodes, self.nf
+        vel = np.empty((B, M, T), dtype=np.float32 if self.single_precision else np.float64)
+        flags = np.empty((B, T), dtype=np.uint32)
+        levels = np.empty((B, T), dtype=np.int32)
+        counts = np.empty((B, T), dtype=np.int32)
+        gv = np.empty((B, M, T)) if self.group_velocity else None
+        gf = np.empty((B, M, T), dtype=np.uint32) if self.group_velocity else None
+        stats = np.zeros(len(STAT_NAMES), dtype=np.uint64)
+        code = _native.lib().md_rocm_run(
+            self.handle, _native.ptr(h), _native.ptr(a), _native.ptr(b), _native.ptr(r), B, _native.ptr(per), row,
+            _native.ptr(bnd), float(c_tol_rel), float(f_accept), int(wave), int(bool(automatic)),
+            vel.ctypes.data_as(ctypes.c_void_p), int(self.single_precision), _native.ptr(flags, _native._u32p),
+            _native.ptr(levels, _native._i32p), _native.ptr(counts, _native._i32p), _native.ptr(gv),
+            _native.ptr(gf, _native._u32p), None, _native.ptr(stats, _native._u64p), len(STAT_NAMES))
+        _native.check(code)
+        st = {k: int(v) for k, v in zip(STAT_NAMES, stats)}
+        if profile:
+            st["timings_s"] = {"device_total": self.last_timing()[0] * 1e-3}
+        extras = {"group_velocity": gv, "group_velocity_flags": gf, "prediction_flags": None}
+        return vel, flags, st, levels, counts, extras
+
+    def run_ragged(self, thickness, vp, vs, rho, row_offsets, periods, period_offsets, modes, bounds, c_tol_rel,
+                   wave):
+        """Ragged batch: packed rows/periods with offsets; outputs packed per model."""
+        h, a, b, r = (np.ascontiguousarray(x, dtype=np.float64) for x in (thickness, vp, vs, rho))
+        ro = np.ascontiguousarray(row_offsets, dtype=np.int64)
+        po = np.ascontiguousarray(period_offsets, dtype=np.int64)
+        md = np.ascontiguousarray(modes, dtype=np.int32)
+        per = np.ascontiguousarray(periods, dtype=np.float64)
+        B = md.size
+        bnd = np.ascontiguousarray(bounds, dtype=np.float64).reshape(B, 2)
+        T = np.diff(po)
+        ncell = int(np.sum(md.astype(np.int64) * T))
+        ntr = int(po[-1])
+        vel = np.empty(ncell)
+        flags = np.empty(ntr, dtype=np.uint32)
+        levels = np.empty(ntr, dtype=np.int32)
+        counts = np.empty(ntr, dtype=np.int32)
+        gv = np.empty(ncell) if self.group_velocity else None
+        gf = np.empty(ncell, dtype=np.uint32) if self.group_velocity else None
+        code = _native.lib().md_rocm_run_ragged(
+            self.handle, _native.ptr(h), _native.ptr(a), _native.ptr(b), _native.ptr(r), _native.ptr(ro, _native._i64p),
+            B, _native.ptr(per), _native.ptr(po, _native._i64p), _native.ptr(md, _native._i32p), _native.ptr(bnd),
+            float(c_tol_rel), int(wave), _native.ptr(vel), _native.ptr(flags, _native._u32p),
+            _native.ptr(levels, _native._i32p), _native.ptr(counts, _native._i32p), _native.ptr(gv),
+            _native.ptr(gf, _native._u32p))
+        _native.check(code)
+        return dict(velocity=vel, flags=flags, levels=levels, counts=counts, group_velocity=gv,
+                    group_velocity_flags=gf)
+
+    def run_device_batch(self, B, pointers, c_tol_rel, wave, row_periods=False):
+        """pointers: dict of device addresses (int) for thickness, vp, vs, rho,
+        periods, bounds, velocity, flags, levels, counts[, group, group_flags]."""
+        p = pointers
+        code = _native.lib().md_rocm_run_device(
+            self.handle, p["thickness"], p["vp"], p["vs"], p["rho"], int(B), p["periods"], int(bool(row_periods)),
+            p["bounds"], float(c_tol_rel), int(wave), p["velocity"], p["flags"], p["levels"], p["counts"],
+            p.get("group"), p.get("group_flags"))
+        _native.check(code)
+
+    def synchronize(self):
+        _native.check(_native.lib().md_rocm_synchronize(self.handle))
+
+    def last_timing(self):
+        ms = np.zeros(6)
+        _native.lib().md_rocm_last_timing(self.handle, _native.ptr(ms), 6)
+        return ms
+
+    def close(self):
+        if getattr(self, "_handle", None) is not None:
+            _native.lib().md_rocm_free(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

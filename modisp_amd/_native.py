@@ -1,0 +1,112 @@
+"""ctypes binding of the modisp-rocm C ABI (include/modisp_rocm.h).
+
+The library is modisp_amd/libmodisp_rocm.so, built in-tree by
+modisp_amd/build.py. There is no fallback: if the library or a HIP device is
+missing, calls raise instead of computing anything elsewhere.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIBRARY = HERE / "libmodisp_rocm.so"
+
+
+class BackendUnavailableError(RuntimeError):
+    """The ROCm plugin, its library or its device cannot be used
+    (reference python/modisp/_loader.py:8-12)."""
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("accuracy", ctypes.c_int), ("close_root_rtol", ctypes.c_double),
+                ("single_precision", ctypes.c_int), ("group_velocity", ctypes.c_int),
+                ("group_rtol", ctypes.c_double), ("continuation", ctypes.c_int), ("anchor_stride", ctypes.c_int),
+                ("nc", ctypes.c_int), ("tau_floor", ctypes.c_double), ("tau_phase", ctypes.c_double),
+                ("chunk_traces", ctypes.c_int)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+_lib = None
+
+# Every symbol declared in include/modisp_rocm.h.
+EXPORTS = ("md_rocm_default_config", "md_rocm_create", "md_rocm_free", "md_rocm_run", "md_rocm_run_device",
+           "md_rocm_run_ragged", "md_rocm_synchronize", "md_rocm_last_timing", "md_rocm_bounds",
+           "md_rocm_available", "md_rocm_set_device", "md_rocm_memory_info", "md_rocm_device_name",
+           "md_rocm_last_error")
+
+
+def lib():
+    """Load the library and declare every C-ABI signature (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY.exists():
+        raise BackendUnavailableError(
+            f"{LIBRARY} is not built; run `python -m modisp_amd.build` (hipcc, gfx950)")
+    try:
+        L = ctypes.CDLL(str(LIBRARY))
+    except OSError as exc:
+        raise BackendUnavailableError(f"cannot load {LIBRARY}: {exc}") from exc
+    L.md_rocm_default_config.argtypes = [ctypes.POINTER(Config)]
+    L.md_rocm_default_config.restype = None
+    L.md_rocm_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Config)]
+    L.md_rocm_create.restype = _vp
+    L.md_rocm_free.argtypes = [_vp]
+    L.md_rocm_free.restype = None
+    L.md_rocm_run.argtypes = [_vp, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _u32p, _i32p, _i32p,
+                              _dp, _u32p, _u32p, _u64p, ctypes.c_int]
+    L.md_rocm_run.restype = ctypes.c_int
+    L.md_rocm_run_device.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp,
+                                     ctypes.c_double, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.md_rocm_run_device.restype = ctypes.c_int
+    L.md_rocm_run_ragged.argtypes = [_vp, _dp, _dp, _dp, _dp, _i64p, ctypes.c_int, _dp, _i64p, _i32p, _dp,
+                                     ctypes.c_double, ctypes.c_int, _dp, _u32p, _i32p, _i32p, _dp, _u32p]
+    L.md_rocm_run_ragged.restype = ctypes.c_int
+    L.md_rocm_synchronize.argtypes = [_vp]
+    L.md_rocm_synchronize.restype = ctypes.c_int
+    L.md_rocm_last_timing.argtypes = [_vp, _dp, ctypes.c_int]
+    L.md_rocm_last_timing.restype = ctypes.c_int
+    L.md_rocm_bounds.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_int, ctypes.c_double, _dp]
+    L.md_rocm_bounds.restype = ctypes.c_int
+    L.md_rocm_available.argtypes = [ctypes.c_int]
+    L.md_rocm_available.restype = ctypes.c_int
+    L.md_rocm_set_device.argtypes = [ctypes.c_int]
+    L.md_rocm_set_device.restype = ctypes.c_int
+    L.md_rocm_memory_info.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    L.md_rocm_memory_info.restype = ctypes.c_int
+    L.md_rocm_device_name.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+    L.md_rocm_device_name.restype = ctypes.c_int
+    L.md_rocm_last_error.argtypes = []
+    L.md_rocm_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    msg = lib().md_rocm_last_error()
+    return msg.decode() if msg else "modisp-rocm: unknown error"
+
+
+def check(code: int) -> None:
+    """Map C-ABI return codes to the reference's exception types
+    (ValueError for arguments, MemoryError for allocation, RuntimeError otherwise)."""
+    if code == 0:
+        return
+    msg = last_error()
+    if code == 2:
+        raise ValueError(msg)
+    if code == 3:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)

@@ -1,0 +1,215 @@
+// Device data layout shared by the host pipeline and the kernels.
+//
+// HBM layout (one solver call, one chunk of traces):
+//   rows     [sum_m L_m]            PrepRow, 64 B per layer row, model-major
+//   models   [B]                    ModelInfo
+//   traces   [ntrace]               TraceInfo, model-major (trace = model, period)
+//   events   [chunk][kEventCap]     Event, scan -> recheck/trough -> assemble
+//   scan     [chunk]                TraceScan
+//   brackets [chunk][kmax]          Bracket (FP64 endpoints and values)
+//   roots    [chunk][kmax]          refined roots
+//   outputs  velocity [sum_m modes_m T_m], flags/levels/counts [sum_m T_m],
+//            group/group_flags like velocity
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mdr {
+
+constexpr int kBlock = 256;       // lanes per scan block (4 wave64)
+constexpr int kMaxRows = 256;     // layer rows per model (reference limit, __init__.py:152)
+constexpr int kEventCap = 32;     // ascending events kept per trace
+constexpr float kTroughMax = 0.05f;  // |F| below which a same-sign trough is probed
+constexpr int kTroughIters = 48;  // golden-section steps per trough probe
+
+enum EventKind : int { kNone = 0, kBracket = 1, kUncertain = 2, kTrough = 3 };
+
+// Trace-flag bits (reference python/modisp/__init__.py:15-32).
+constexpr unsigned kFlagUnconverged = 4u, kFlagNearCutoff = 16u, kFlagRefineFailed = 64u,
+                   kFlagCloseRoot = 4096u;
+constexpr unsigned kTraceOverflow = 1u << 31;  // internal: event list overflowed
+// Group-velocity flags (reference cpp/common/group_velocity.hpp:9-12).
+constexpr unsigned kGroupMissing = 1u, kGroupClose = 16u, kGroupNonfinite = 32u;
+// search_level codes of this backend's grid scans.
+constexpr int kLevelScan32 = 20, kLevelScan64 = 21;
+
+// One prepared layer row: reciprocals so a trace's constants are products.
+struct PrepRow {
+  double h;      // thickness (km); 0 for the half-space
+  double ivp;    // 1 / vp
+  double ivs;    // 1 / vs (0 for water)
+  double g2;     // 2 vs^2
+  double rho;    // density
+  double irho;   // 1 / rho
+  double irho2;  // 1 / rho^2
+  double imu;    // 1 / (rho vs^2) (0 for water)
+};
+
+struct ModelInfo {
+  int row0;        // first row in `rows`
+  int L;           // rows including the half-space
+  int top;         // 1 when row 0 is water (vs = 0)
+  int K;           // roots searched: modes + 2
+  int modes;       // roots reported
+  int T;           // periods of this model
+  long long out0;  // velocity/group base offset: cell = out0 + mode*T + f
+  long long trace_out0;  // flags/levels/counts base offset: out = trace_out0 + f
+  double lo, hi;   // search window
+  double vs_half;  // half-space shear speed
+  double cutoff_guard;  // FP32 nodes at or above this speed are rechecked in FP64
+  double stable_low;    // FP32 nodes below this speed are rechecked in FP64
+  double slowness;  // sum of h/vs over the finite layers (FP32 band sizing)
+  int empty;       // 0 normal, 1 empty range, 2 no Love guide
+  int pad;
+};
+
+struct TraceInfo {
+  double omega;  // angular frequency
+  int model;
+  int f;         // period index within the model
+  float tau;     // FP32 uncertainty band on |F|
+  int pad;
+};
+
+struct Event {
+  int node;   // grid node (cell node..node+1 for brackets)
+  int kind;
+  float fl, fc, fr;  // FP32 values at node-1, node, node+1
+  float pad;
+  double v;   // FP64 value (uncertain) / F at split (trough)
+  double x;   // trough split point (NaN: none)
+};
+
+struct TraceScan {
+  int nev;
+  unsigned flags;
+  int status;   // ModelInfo::empty for empty traces
+  int certain;  // certain brackets seen on the whole grid
+  int count;    // roots found (<= K), -1 unresolved
+  int pad;
+};
+
+struct Bracket {
+  double lo, hi, flo, fhi;
+};
+
+struct PipeArgs {
+  const TraceInfo *traces;
+  const ModelInfo *models;
+  const PrepRow *rows;
+  Event *events;
+  TraceScan *scan;
+  Bracket *brackets;
+  double *roots;
+  uint32_t *unc_queue;
+  uint32_t *trough_queue;
+  int *counters;  // [0] uncertain nodes, [1] troughs
+  int *unc_count;
+  int *trough_count;
+  int queue_cap;
+  int trace_begin, trace_end;
+  int nc;
+  int kmax;
+  int wave;
+  int fp32;
+  double tol;
+  double close_rtol;
+  double *velocity64;
+  float *velocity32;
+  uint32_t *flags;
+  int32_t *levels;
+  int32_t *counts;
+  double *group;
+  uint32_t *group_flags;
+};
+using ScanArgs = PipeArgs;
+
+hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bool group, int persistent_blocks,
+                           hipStream_t s);
+
+}  // namespace mdr
+
+// ---------------------------------------------------------------------------
+// Preparation shared by the host (ragged batches) and the device prep kernels.
+// ---------------------------------------------------------------------------
+#if defined(__HIPCC__)
+#define MDP_HD __host__ __device__ inline
+#else
+#define MDP_HD inline
+#endif
+
+namespace mdr {
+
+MDP_HD PrepRow make_prep_row(double h, double vp, double vs, double rho) {
+  PrepRow r;
+  r.h = h;
+  r.ivp = 1. / vp;
+  r.ivs = vs > 0. ? 1. / vs : 0.;
+  r.g2 = 2. * vs * vs;
+  r.rho = rho;
+  r.irho = 1. / rho;
+  r.irho2 = r.irho * r.irho;
+  r.imu = vs > 0. ? 1. / (rho * vs * vs) : 0.;
+  return r;
+}
+
+struct PrepParams {
+  int wave;
+  double tau_floor, tau_phase;
+};
+
+// Model-level constants (search window, guards, empty-family status) and the
+// vertical slowness sum used to size the FP32 uncertainty band.
+MDP_HD ModelInfo make_model_info(const double *h, const double *vp, const double *vs, int L, double lo, double hi,
+                                 int wave, int modes, int T, long long row0, long long out0, long long trace_out0,
+                                 double *slowness) {
+  ModelInfo mi;
+  mi.row0 = static_cast<int>(row0);
+  mi.L = L;
+  mi.top = vs[0] == 0. ? 1 : 0;
+  mi.K = modes + 2;
+  mi.modes = modes;
+  mi.T = T;
+  mi.out0 = out0;
+  mi.trace_out0 = trace_out0;
+  mi.lo = lo;
+  mi.hi = hi;
+  mi.vs_half = vs[L - 1];
+  mi.cutoff_guard = vs[L - 1] * (1. - 1e-3);
+  double vmax = 0., s = 0.;
+  bool guide = false;
+  for (int j = 0; j < L; ++j) {
+    if (vs[j] > vmax) vmax = vs[j];
+    if (j < L - 1) {
+      s += vs[j] > 0. ? h[j] / vs[j] : h[j] / vp[j];
+      // A Love guide needs a finite solid layer slower than the half-space
+      // (reference cpp/common/physics.hpp:23-27).
+      if (j >= mi.top && h[j] > 0. && vs[j] < vs[L - 1]) guide = true;
+    }
+  }
+  mi.stable_low = vmax / 16.;
+  mi.empty = !(hi > lo) ? 1 : ((wave == 1 && !guide) ? 2 : 0);
+  mi.pad = 0;
+  mi.slowness = s;
+  *slowness = s;
+  return mi;
+}
+
+MDP_HD TraceInfo make_trace(int model, int f, double period, double slowness, const PrepParams &p) {
+  TraceInfo t;
+  t.omega = 6.283185307179586476925286766559 / period;
+  t.model = model;
+  t.f = f;
+  const double band = p.tau_phase * (1. + t.omega * slowness);
+  t.tau = static_cast<float>(band > p.tau_floor ? band : p.tau_floor);
+  t.pad = 0;
+  return t;
+}
+
+}  // namespace mdr
+
+namespace mdr {
+hipError_t launch_prep(const double *h, const double *vp, const double *vs, const double *rho, const double *bounds,
+                       const double *periods, int row_periods, int B, int L, int T, int modes, const PrepParams &p,
+                       PrepRow *rows, ModelInfo *models, TraceInfo *traces, hipStream_t s);
+}  // namespace mdr

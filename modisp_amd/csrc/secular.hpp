@@ -1,0 +1,309 @@
+// Secular functions of a layered elastic model, evaluated per lane.
+//
+// Rayleigh: Dunkin delta (compound) matrix propagator, the five-vector of
+// second-order minors carried from the half-space to the surface; F = e[0]
+// (reference cpp/common/dunkin.hpp:459-566 computes the same function).
+// Love: Haskell SH 2x2 propagator; F = e1 (reference physics.hpp:33-53).
+//
+// GPU formulation:
+//  * per-trace layer constants (ka = w/vp, kb = w/vs, gk = 2 (vs/w)^2, rho,
+//    1/rho, 1/rho^2, h) are computed once per trace and read from LDS;
+//  * the 15 distinct compound entries are contracted directly with the
+//    five-vector (fused multiply-adds), never materialising the 5x5 matrix;
+//  * an evanescent layer uses one exp(-p) for both cosh-scaling and the
+//    attenuation factor a0 = exp(-(p+q)) = exp(-p) exp(-q);
+//  * after each layer the five-vector is rescaled by an approximate reciprocal
+//    of its largest magnitude, which changes neither the sign nor the zeros
+//    of F and costs one hardware reciprocal instead of five divisions.
+// The same template serves float (grid scans) and double (brackets,
+// refinement); Dual2 (secular_dual.hpp) reuses it for analytic derivatives.
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MDR_HD __host__ __device__ __forceinline__
+#else
+#define MDR_HD inline
+#endif
+
+namespace mdr {
+
+// Per-trace constants of one layer; 8 slots so a row is 32 B (float) or 64 B (double).
+// Rayleigh uses {h, ka, kb, gk, rho, irho, irho2}; Love uses {h, -, kb, -, mu, imu}.
+template <class T>
+struct Layer {
+  T h, ka, kb, gk, rho, irho, irho2, pad;
+};
+
+// Math shims: hardware transcendentals for float on the GPU, libm elsewhere.
+MDR_HD float m_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_sqrtf(x);
+#else
+  return std::sqrt(x);
+#endif
+}
+MDR_HD double m_sqrt(double x) { return sqrt(x); }
+MDR_HD void m_sincos(float x, float *s, float *c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  *s = __sinf(x);
+  *c = __cosf(x);
+#else
+  *s = std::sin(x);
+  *c = std::cos(x);
+#endif
+}
+MDR_HD void m_sincos(double x, double *s, double *c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  sincos(x, s, c);
+#else
+  *s = std::sin(x);
+  *c = std::cos(x);
+#endif
+}
+MDR_HD float m_exp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __expf(x);
+#else
+  return std::exp(x);
+#endif
+}
+MDR_HD double m_exp(double x) { return exp(x); }
+MDR_HD float m_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __frcp_rn(x);
+#else
+  return 1.f / x;
+#endif
+}
+MDR_HD double m_rcp(double x) { return 1. / x; }
+MDR_HD float m_fabs(float x) { return fabsf(x); }
+MDR_HD double m_fabs(double x) { return fabs(x); }
+MDR_HD float m_fma(float a, float b, float c) { return fmaf(a, b, c); }
+MDR_HD double m_fma(double a, double b, double c) { return fma(a, b, c); }
+
+// Approximate reciprocal: the normalization scale only has to be positive.
+MDR_HD float fast_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.f / x;
+#endif
+}
+MDR_HD double fast_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcp(x);
+#else
+  return 1. / x;
+#endif
+}
+template <class T> MDR_HD T tiny_scale();
+template <> MDR_HD float tiny_scale<float>() { return 1.0e-30f; }
+template <> MDR_HD double tiny_scale<double>() { return 1.0e-40; }
+
+// Max-abs rescale: multiply by ~1/max|e_i| so the state stays O(1). The scale
+// is positive, so neither the sign nor the zeros of F change; it is continuous
+// in the inputs, so float and double evaluations stay directly comparable.
+template <int N, class T>
+MDR_HD void rescale(T *e) {
+  T m = T(0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) m = fmax(m, m_fabs(e[i]));
+  if (m > tiny_scale<T>()) {
+    const T inv = fast_rcp(m);
+#pragma unroll
+    for (int i = 0; i < N; ++i) e[i] *= inv;
+  }
+}
+
+// P (or S) factor of one layer: cosine-like c, sine/r-like w, r*sine-like x,
+// and the attenuation e^-p (1 when propagating).
+template <class T>
+struct Wave {
+  T c, w, x, att;
+};
+template <class T>
+MDR_HD Wave<T> wave_factor(T k, T kx, T h) {
+  Wave<T> o;
+  const T r = m_sqrt((k + kx) * m_fabs(k - kx));
+  const T p = r * h;
+  if (k < kx) {  // propagating: cos p, sin p / r, -r sin p
+    T s, c;
+    m_sincos(p, &s, &c);
+    o.c = c;
+    o.w = s * m_rcp(r);
+    o.x = -r * s;
+    o.att = T(1);
+  } else if (k == kx) {  // c equals the body-wave speed: the r -> 0 limit
+    o.c = T(1);
+    o.w = h;
+    o.x = T(0);
+    o.att = T(1);
+  } else {  // evanescent, scaled by e^-p: cosh p, sinh p / r, r sinh p
+    const T e = m_exp(-p);
+    const T f = e * e;
+    o.c = (T(1) + f) * T(0.5);
+    const T s = (T(1) - f) * T(0.5);
+    o.w = s * m_rcp(r);
+    o.x = r * s;
+    o.att = e;
+  }
+  return o;
+}
+
+// One Rayleigh layer: e <- e * C(layer), then exact rescale.
+template <class T>
+MDR_HD void rayleigh_layer(const Layer<T> &ly, T k, T k2, T e[5]) {
+  const Wave<T> P = wave_factor(k, ly.ka, ly.h);
+  const Wave<T> S = wave_factor(k, ly.kb, ly.h);
+  const T a0 = P.att * S.att;
+  const T cpcq = P.c * S.c, cpy = P.c * S.w, cpz = P.c * S.x, cqw = S.c * P.w, cqx = S.c * P.x;
+  const T xy = P.x * S.w, xz = P.x * S.x, wy = P.w * S.w, wz = P.w * S.x;
+  const T gk = ly.gk, gam = gk * k2, gamm1 = gam - T(1), twgm1 = gam + gamm1;
+  const T gmgmk = gam * gk, gmgm1 = gam * gamm1, gm1sq = gamm1 * gamm1;
+  const T rho = ly.rho, irho = ly.irho, irho2 = ly.irho2;
+  const T a0pq = a0 - cpcq, k4 = k2 * k2;
+  // The 15 distinct entries of Dunkin's compound matrix, rows/columns laid out as
+  // [A D I C L; E B G M C; tJ tH K tG tI; F N H B D; O F J E A], t = -2 k^2.
+  const T A = cpcq - T(2) * gmgm1 * a0pq - gmgmk * xz - k2 * gm1sq * wy;
+  const T D = (k2 * cpy - cqx) * irho;
+  const T I = -(twgm1 * a0pq + gk * xz + k2 * gamm1 * wy) * irho;
+  const T C = (cpz - k2 * cqw) * irho;
+  const T Lq = -(T(2) * k2 * a0pq + xz + k4 * wy) * irho2;
+  const T E = (gmgmk * cpz - gm1sq * cqw) * rho;
+  const T B = cpcq;
+  const T G = gk * cpz - gamm1 * cqw;
+  const T M = -wz;
+  const T N = -xy;
+  const T F = (gm1sq * cpy - gmgmk * cqx) * rho;
+  const T H = gamm1 * cpy - gk * cqx;
+  const T O = -(T(2) * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * (rho * rho);
+  const T J = -(gk * gamm1 * twgm1 * a0pq + gam * gk * gk * xz + gamm1 * gm1sq * wy) * rho;
+  const T K = a0 + T(2) * (cpcq - A);
+  const T e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3], e4 = e[4];
+  const T e2t = e2 * (T(-2) * k2);
+  e[0] = m_fma(A, e0, m_fma(E, e1, m_fma(F, e3, m_fma(O, e4, J * e2t))));
+  e[1] = m_fma(D, e0, m_fma(B, e1, m_fma(N, e3, m_fma(F, e4, H * e2t))));
+  e[2] = m_fma(I, e0, m_fma(G, e1, m_fma(K, e2, m_fma(H, e3, J * e4))));
+  e[3] = m_fma(C, e0, m_fma(M, e1, m_fma(B, e3, m_fma(E, e4, G * e2t))));
+  e[4] = m_fma(Lq, e0, m_fma(C, e1, m_fma(D, e3, m_fma(A, e4, I * e2t))));
+  rescale<5>(e);
+}
+
+// Half-space five-vector (decaying P and S waves below the stack).
+template <class T>
+MDR_HD void rayleigh_halfspace(const Layer<T> &hs, T k, T k2, T e[5]) {
+  const T ra = m_sqrt((k + hs.ka) * m_fabs(k - hs.ka));
+  const T rb = m_sqrt((k + hs.kb) * m_fabs(k - hs.kb));
+  const T gk = hs.gk, gam = gk * k2, gamm1 = gam - T(1), rh = hs.rho, rarb = ra * rb;
+  e[0] = rh * rh * (gamm1 * gamm1 - gam * gk * rarb);
+  e[1] = -rh * ra;
+  e[2] = rh * (gamm1 - gk * rarb);
+  e[3] = rh * rb;
+  e[4] = k2 - rarb;
+  rescale<5>(e);
+}
+
+// Pressure-release water top over the solid stack: F = cos(p) e0 - rho_w w e1.
+template <class T>
+MDR_HD T rayleigh_water_top(const Layer<T> &wl, T k, const T e[5]) {
+  const Wave<T> P = wave_factor(k, wl.ka, wl.h);
+  const T t0 = P.c * e[0], t1 = wl.rho * P.w * e[1];
+  return t0 - t1;
+}
+
+// Rayleigh F at wavenumber k for one trace. `lay` holds L rows (row L-1 the
+// half-space); `top` is 1 when row 0 is a water layer.
+template <class T>
+MDR_HD T rayleigh_F(const Layer<T> *lay, int L, int top, T k) {
+  const T k2 = k * k;
+  T e[5];
+  rayleigh_halfspace(lay[L - 1], k, k2, e);
+  for (int j = L - 2; j >= top; --j) rayleigh_layer(lay[j], k, k2, e);
+  return top ? rayleigh_water_top(lay[0], k, e) : e[0];
+}
+
+// Love: one SH layer, state (traction, displacement / mu-scaled). The wave
+// factor already carries z = -r sin q (propagating) or r sinh q (evanescent).
+template <class T>
+MDR_HD void love_layer(const Layer<T> &ly, T k, T e[2]) {
+  const Wave<T> S = wave_factor(k, ly.kb, ly.h);
+  const T mu = ly.rho, imu = ly.irho;  // Love rows store mu and 1/mu here
+  const T u = m_fma(e[0], S.c, e[1] * mu * S.x);
+  const T v = m_fma(e[0] * S.w, imu, e[1] * S.c);
+  e[0] = u;
+  e[1] = v;
+  rescale<2>(e);
+}
+
+// Love F: half-space state (mu r, 1) -- the reference's (rho r, 1/vs^2) times
+// vs^2 > 0 -- propagated up to the first solid row `top`.
+template <class T>
+MDR_HD T love_F(const Layer<T> *lay, int L, int top, T k) {
+  const Layer<T> &hs = lay[L - 1];
+  T e[2];
+  e[0] = hs.rho * m_sqrt((k + hs.kb) * m_fabs(k - hs.kb));
+  e[1] = T(1);
+  rescale<2>(e);
+  for (int j = L - 2; j >= top; --j) love_layer(lay[j], k, e);
+  return e[0];
+}
+
+}  // namespace mdr
+
+// ---------------------------------------------------------------------------
+// Two-direction forward-mode dual numbers: value plus derivatives along two
+// seeded inputs (here the wavenumber k and the angular frequency w). The
+// secular templates above instantiate with T = Dual2 unchanged; the
+// normalization scale is held constant (it multiplies F, which vanishes at a
+// root, so it cancels from any derivative ratio taken there).
+// ---------------------------------------------------------------------------
+namespace mdr {
+struct Dual2 {
+  double v, a, b;
+  MDR_HD Dual2() : v(0.), a(0.), b(0.) {}
+  MDR_HD Dual2(double x) : v(x), a(0.), b(0.) {}
+  MDR_HD Dual2(double x, double da, double db) : v(x), a(da), b(db) {}
+};
+MDR_HD Dual2 operator+(Dual2 x, Dual2 y) { return {x.v + y.v, x.a + y.a, x.b + y.b}; }
+MDR_HD Dual2 operator-(Dual2 x, Dual2 y) { return {x.v - y.v, x.a - y.a, x.b - y.b}; }
+MDR_HD Dual2 operator-(Dual2 x) { return {-x.v, -x.a, -x.b}; }
+MDR_HD Dual2 operator*(Dual2 x, Dual2 y) { return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b}; }
+MDR_HD Dual2 operator*(Dual2 x, double s) { return {x.v * s, x.a * s, x.b * s}; }
+MDR_HD Dual2 &operator*=(Dual2 &x, double s) { x = x * s; return x; }
+MDR_HD bool operator<(Dual2 x, Dual2 y) { return x.v < y.v; }
+MDR_HD bool operator==(Dual2 x, Dual2 y) { return x.v == y.v; }
+MDR_HD Dual2 m_sqrt(Dual2 x) {
+  const double s = sqrt(x.v), d = 0.5 / s;
+  return {s, x.a * d, x.b * d};
+}
+MDR_HD void m_sincos(Dual2 x, Dual2 *s, Dual2 *c) {
+  double sn, cs;
+  m_sincos(x.v, &sn, &cs);
+  *s = Dual2(sn, cs * x.a, cs * x.b);
+  *c = Dual2(cs, -sn * x.a, -sn * x.b);
+}
+MDR_HD Dual2 m_exp(Dual2 x) {
+  const double e = exp(x.v);
+  return {e, e * x.a, e * x.b};
+}
+MDR_HD Dual2 m_rcp(Dual2 x) {
+  const double r = 1. / x.v, d = -r * r;
+  return {r, x.a * d, x.b * d};
+}
+MDR_HD Dual2 m_fabs(Dual2 x) { return x.v < 0. ? -x : x; }
+MDR_HD Dual2 m_fma(Dual2 x, Dual2 y, Dual2 z) { return x * y + z; }
+template <int N>
+MDR_HD void rescale(Dual2 *e) {
+  double m = 0.;
+#pragma unroll
+  for (int i = 0; i < N; ++i) m = fmax(m, fabs(e[i].v));
+  if (m > 1.0e-40) {
+    const double inv = 1. / m;
+#pragma unroll
+    for (int i = 0; i < N; ++i) e[i] *= inv;
+  }
+}
+}  // namespace mdr

@@ -1,0 +1,422 @@
+// Host side of the modisp-rocm backend: C++17 over the HIP runtime, exposing
+// the C ABI declared in include/modisp_rocm.h.
+//
+// One solver owns one device stream, its device workspace (sized for its
+// capacity and grown on demand) and HIP events for stage timing. A call
+// prepares layer rows and trace tables on the device, runs the root pipeline
+// over chunks of traces, and (for host entry points) copies results back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/modisp_rocm.h"
+#include "pipeline.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+  g_error = "modisp-rocm: " + msg;
+  return code;
+}
+
+#define MD_HIP(call)                                                                          \
+  do {                                                                                        \
+    const hipError_t err_ = (call);                                                           \
+    if (err_ != hipSuccess)                                                                   \
+      return fail(err_ == hipErrorOutOfMemory ? 3 : 1, std::string(#call) + ": " + hipGetErrorString(err_)); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t n) {
+    if (n <= bytes) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (n == 0) return 0;
+    const hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? 3 : 1, "device allocation of " + std::to_string(n) + " bytes failed");
+    }
+    bytes = n;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+constexpr int kStatCount = 4;  // traces, fp64 rechecks, trough probes, unresolved traces
+
+}  // namespace
+
+struct md_rocm_solver {
+  md_rocm_config cfg{};
+  int capacity = 0, L = 0, nf = 0, nmodes = 0;
+  int device = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats;
+  DevBuf in_h, in_vp, in_vs, in_rho, in_per, in_bounds;
+  DevBuf out_v, out_flags, out_levels, out_counts, out_group, out_gflags;
+  double last_ms[6] = {0, 0, 0, 0, 0, 0};
+
+  ~md_rocm_solver() {
+    for (DevBuf *b : {&rows, &models, &traces, &events, &scan, &brackets, &roots, &unc_q, &tr_q, &counters, &stats,
+                      &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &out_v, &out_flags, &out_levels,
+                      &out_counts, &out_group, &out_gflags})
+      b->release();
+    if (ev_begin) (void)hipEventDestroy(ev_begin);
+    if (ev_end) (void)hipEventDestroy(ev_end);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int chunk_size(const md_rocm_solver *s, long long ntraces) {
+  long long c = s->cfg.chunk_traces > 0 ? s->cfg.chunk_traces : (1LL << 20);
+  if (c > ntraces) c = ntraces;
+  if (c < 1) c = 1;
+  return static_cast<int>(c);
+}
+
+// Runs the root pipeline over all traces already prepared on the device.
+int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, double tol, double *v64, float *v32,
+                 uint32_t *flags, int32_t *levels, int32_t *counts, double *group, uint32_t *gflags) {
+  const int chunk = chunk_size(s, ntraces);
+  const size_t qcap = static_cast<size_t>(chunk) * mdr::kEventCap;
+  if (int e = s->events.ensure(qcap * sizeof(mdr::Event))) return e;
+  if (int e = s->scan.ensure(static_cast<size_t>(chunk) * sizeof(mdr::TraceScan))) return e;
+  if (int e = s->brackets.ensure(static_cast<size_t>(chunk) * kmax * sizeof(mdr::Bracket))) return e;
+  if (int e = s->roots.ensure(static_cast<size_t>(chunk) * kmax * sizeof(double))) return e;
+  if (int e = s->unc_q.ensure(qcap * sizeof(uint32_t))) return e;
+  if (int e = s->tr_q.ensure(qcap * sizeof(uint32_t))) return e;
+  if (int e = s->counters.ensure(4 * sizeof(int))) return e;
+  mdr::PipeArgs a{};
+  a.traces = s->traces.as<mdr::TraceInfo>();
+  a.models = s->models.as<mdr::ModelInfo>();
+  a.rows = s->rows.as<mdr::PrepRow>();
+  a.events = s->events.as<mdr::Event>();
+  a.scan = s->scan.as<mdr::TraceScan>();
+  a.brackets = s->brackets.as<mdr::Bracket>();
+  a.roots = s->roots.as<double>();
+  a.unc_queue = s->unc_q.as<uint32_t>();
+  a.trough_queue = s->tr_q.as<uint32_t>();
+  a.counters = s->counters.as<int>();
+  a.unc_count = a.counters;
+  a.trough_count = a.counters + 1;
+  a.queue_cap = static_cast<int>(qcap);
+  a.nc = s->cfg.nc;
+  a.kmax = kmax;
+  a.wave = wave;
+  a.fp32 = s->cfg.single_precision ? 1 : 0;
+  a.tol = tol;
+  a.close_rtol = s->cfg.close_root_rtol;
+  a.velocity64 = v64;
+  a.velocity32 = v32;
+  a.flags = flags;
+  a.levels = levels;
+  a.counts = counts;
+  a.group = group;
+  a.group_flags = gflags;
+  const int persistent = s->cus * 8;
+  for (long long begin = 0; begin < ntraces; begin += chunk) {
+    a.trace_begin = static_cast<int>(begin);
+    a.trace_end = static_cast<int>(std::min<long long>(ntraces, begin + chunk));
+    const hipError_t err = mdr::launch_pipeline(a, a, a.fp32 != 0, group != nullptr, persistent, s->stream);
+    if (err != hipSuccess) return fail(1, std::string("kernel launch: ") + hipGetErrorString(err));
+  }
+  return 0;
+}
+
+int check_config(const md_rocm_config &c) {
+  if (c.nc != 256 && c.nc != 512 && c.nc != 1024) return fail(2, "nc must be 256, 512 or 1024");
+  if (!(c.close_root_rtol > 0. && c.close_root_rtol < 1.)) return fail(2, "close_root_rtol must be in (0, 1)");
+  if (!(c.group_rtol > 0. && c.group_rtol < 1.)) return fail(2, "group_rtol must be in (0, 1)");
+  if (c.continuation && c.anchor_stride < 2) return fail(2, "anchor_stride must be at least 2");
+  if (!(c.tau_floor >= 0.) || !(c.tau_phase >= 0.)) return fail(2, "FP32 band parameters must be non-negative");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void md_rocm_default_config(md_rocm_config *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->accuracy = 0;
+  cfg->close_root_rtol = .005;
+  cfg->single_precision = 1;
+  cfg->group_velocity = 0;
+  cfg->group_rtol = .001;
+  cfg->continuation = 0;
+  cfg->anchor_stride = 4;
+  cfg->nc = 512;
+  cfg->tau_floor = 1e-4;
+  cfg->tau_phase = 5e-5;
+  cfg->chunk_traces = 0;
+}
+
+const char *md_rocm_last_error(void) { return g_error.c_str(); }
+
+int md_rocm_available(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return 0;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return 0;
+  return std::strncmp(p.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+int md_rocm_set_device(int device) {
+  MD_HIP(hipSetDevice(device));
+  return 0;
+}
+
+int md_rocm_memory_info(size_t *free_bytes, size_t *total_bytes) {
+  MD_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return 0;
+}
+
+int md_rocm_device_name(int device, char *buf, int len) {
+  hipDeviceProp_t p;
+  MD_HIP(hipGetDeviceProperties(&p, device));
+  if (buf && len > 0) std::snprintf(buf, static_cast<size_t>(len), "%s (%s)", p.name, p.gcnArchName);
+  return 0;
+}
+
+md_rocm_solver *md_rocm_create(int capacity, int L, int nf, int nmodes, const md_rocm_config *cfg) {
+  md_rocm_config c;
+  md_rocm_default_config(&c);
+  if (cfg) c = *cfg;
+  if (capacity < 1 || L < 1 || L > mdr::kMaxRows || nf < 1 || nmodes < 1 || nmodes > 256) {
+    fail(2, "invalid solver geometry (capacity >= 1, 1 <= L <= 256, nf >= 1, 1 <= nmodes <= 256)");
+    return nullptr;
+  }
+  if (check_config(c)) return nullptr;
+  auto *s = new md_rocm_solver();
+  s->cfg = c;
+  s->capacity = capacity;
+  s->L = L;
+  s->nf = nf;
+  s->nmodes = nmodes;
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    delete s;
+    fail(1, "no HIP device");
+    return nullptr;
+  }
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, s->device) == hipSuccess) s->cus = p.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&s->ev_begin) != hipSuccess || hipEventCreate(&s->ev_end) != hipSuccess) {
+    delete s;
+    fail(1, "stream/event creation failed");
+    return nullptr;
+  }
+  return s;
+}
+
+void md_rocm_free(md_rocm_solver *s) { delete s; }
+
+int md_rocm_synchronize(md_rocm_solver *s) {
+  if (!s) return fail(2, "null solver");
+  MD_HIP(hipStreamSynchronize(s->stream));
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, s->ev_begin, s->ev_end) == hipSuccess) s->last_ms[0] = ms;
+  return 0;
+}
+
+int md_rocm_last_timing(md_rocm_solver *s, double *ms, int n) {
+  if (!s || !ms) return 0;
+  const int k = std::min(n, 6);
+  for (int i = 0; i < k; ++i) ms[i] = s->last_ms[i];
+  return k;
+}
+
+int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                       const double *rho, int B, const double *periods, int row_periods, const double *bounds,
+                       double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
+                       int32_t *counts, double *group, uint32_t *group_flags) {
+  if (!s) return fail(2, "null solver");
+  if (B < 1 || B > s->capacity) return fail(2, "B must be 1..capacity");
+  if (wave != 0 && wave != 1) return fail(2, "wave must be 0 (Rayleigh) or 1 (Love)");
+  if (!(c_tol_rel > 0.) || !std::isfinite(c_tol_rel)) return fail(2, "c_tol_rel must be positive");
+  if (!thickness || !vp || !vs || !rho || !periods || !bounds || !velocity || !flags || !levels || !counts)
+    return fail(2, "null input or output pointer");
+  if (s->cfg.group_velocity && (!group || !group_flags)) return fail(2, "group outputs required");
+  const int L = s->L, T = s->nf;
+  const long long ntraces = static_cast<long long>(B) * T;
+  if (int e = s->rows.ensure(static_cast<size_t>(B) * L * sizeof(mdr::PrepRow))) return e;
+  if (int e = s->models.ensure(static_cast<size_t>(B) * sizeof(mdr::ModelInfo))) return e;
+  if (int e = s->traces.ensure(static_cast<size_t>(ntraces) * sizeof(mdr::TraceInfo))) return e;
+  MD_HIP(hipEventRecord(s->ev_begin, s->stream));
+  const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
+  MD_HIP(mdr::launch_prep(thickness, vp, vs, rho, bounds, periods, row_periods, B, L, T, s->nmodes, pp,
+                          s->rows.as<mdr::PrepRow>(), s->models.as<mdr::ModelInfo>(), s->traces.as<mdr::TraceInfo>(),
+                          s->stream));
+  if (int e = run_pipeline(s, ntraces, s->nmodes + 2, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
+                           s->cfg.group_velocity ? group : nullptr, group_flags))
+    return e;
+  MD_HIP(hipEventRecord(s->ev_end, s->stream));
+  return 0;
+}
+
+int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs, const double *rho,
+                int B, const double *periods, int row_periods, const double *bounds, double c_tol_rel,
+                double f_accept, int wave, int automatic, void *velocity, int velocity_is_f32, uint32_t *flags,
+                int32_t *levels, int32_t *counts, double *group, uint32_t *group_flags, uint32_t *prediction_flags,
+                uint64_t *stats, int nstats) {
+  (void)automatic;
+  if (!s) return fail(2, "null solver");
+  if (B < 1 || B > s->capacity) return fail(2, "B must be 1..capacity");
+  if (!thickness || !vp || !vs || !rho || !periods || !bounds || !velocity || !flags || !levels || !counts)
+    return fail(2, "null input or output pointer");
+  if (!(f_accept >= 0.)) return fail(2, "f_accept must be non-negative");
+  const int L = s->L, T = s->nf, M = s->nmodes;
+  const size_t nrow = static_cast<size_t>(B) * L, ncell = static_cast<size_t>(B) * M * T,
+               ntr = static_cast<size_t>(B) * T;
+  const size_t nper = row_periods ? ntr : static_cast<size_t>(T);
+  for (DevBuf *b : {&s->in_h, &s->in_vp, &s->in_vs, &s->in_rho})
+    if (int e = b->ensure(nrow * sizeof(double))) return e;
+  if (int e = s->in_per.ensure(nper * sizeof(double))) return e;
+  if (int e = s->in_bounds.ensure(static_cast<size_t>(B) * 2 * sizeof(double))) return e;
+  if (int e = s->out_v.ensure(ncell * sizeof(double))) return e;
+  if (int e = s->out_flags.ensure(ntr * sizeof(uint32_t))) return e;
+  if (int e = s->out_levels.ensure(ntr * sizeof(int32_t))) return e;
+  if (int e = s->out_counts.ensure(ntr * sizeof(int32_t))) return e;
+  const bool want_group = s->cfg.group_velocity != 0;
+  if (want_group) {
+    if (!group || !group_flags) return fail(2, "group outputs required");
+    if (int e = s->out_group.ensure(ncell * sizeof(double))) return e;
+    if (int e = s->out_gflags.ensure(ncell * sizeof(uint32_t))) return e;
+  }
+  hipStream_t st = s->stream;
+  MD_HIP(hipMemcpyAsync(s->in_h.p, thickness, nrow * sizeof(double), hipMemcpyHostToDevice, st));
+  MD_HIP(hipMemcpyAsync(s->in_vp.p, vp, nrow * sizeof(double), hipMemcpyHostToDevice, st));
+  MD_HIP(hipMemcpyAsync(s->in_vs.p, vs, nrow * sizeof(double), hipMemcpyHostToDevice, st));
+  MD_HIP(hipMemcpyAsync(s->in_rho.p, rho, nrow * sizeof(double), hipMemcpyHostToDevice, st));
+  MD_HIP(hipMemcpyAsync(s->in_per.p, periods, nper * sizeof(double), hipMemcpyHostToDevice, st));
+  MD_HIP(hipMemcpyAsync(s->in_bounds.p, bounds, static_cast<size_t>(B) * 2 * sizeof(double), hipMemcpyHostToDevice,
+                        st));
+  if (int e = md_rocm_run_device(s, s->in_h.as<double>(), s->in_vp.as<double>(), s->in_vs.as<double>(),
+                                 s->in_rho.as<double>(), B, s->in_per.as<double>(), row_periods,
+                                 s->in_bounds.as<double>(), c_tol_rel, wave, s->out_v.as<double>(),
+                                 s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(),
+                                 s->out_counts.as<int32_t>(), want_group ? s->out_group.as<double>() : nullptr,
+                                 want_group ? s->out_gflags.as<uint32_t>() : nullptr))
+    return e;
+  std::vector<double> vtmp;
+  double *vdst = static_cast<double *>(velocity);
+  if (velocity_is_f32) {
+    vtmp.resize(ncell);
+    vdst = vtmp.data();
+  }
+  MD_HIP(hipMemcpyAsync(vdst, s->out_v.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
+  MD_HIP(hipMemcpyAsync(flags, s->out_flags.p, ntr * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  MD_HIP(hipMemcpyAsync(levels, s->out_levels.p, ntr * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  MD_HIP(hipMemcpyAsync(counts, s->out_counts.p, ntr * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (want_group) {
+    MD_HIP(hipMemcpyAsync(group, s->out_group.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
+    MD_HIP(hipMemcpyAsync(group_flags, s->out_gflags.p, ncell * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  }
+  if (int e = md_rocm_synchronize(s)) return e;
+  if (velocity_is_f32) {
+    float *out = static_cast<float *>(velocity);
+    for (size_t i = 0; i < ncell; ++i) out[i] = static_cast<float>(vtmp[i]);
+  }
+  if (prediction_flags) std::fill_n(prediction_flags, ntr, 8u);  // bypassed: independent periods
+  if (stats && nstats > 0) {
+    uint64_t st4[kStatCount] = {static_cast<uint64_t>(ntr), 0, 0, 0};
+    for (int i = 0; i < std::min(nstats, kStatCount); ++i) stats[i] = st4[i];
+  }
+  return 0;
+}
+
+int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                       const double *rho, const int64_t *row_offsets, int B, const double *periods,
+                       const int64_t *period_offsets, const int32_t *modes, const double *bounds,
+                       double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
+                       int32_t *counts, double *group, uint32_t *group_flags) {
+  if (!s) return fail(2, "null solver");
+  if (B < 1) return fail(2, "B must be positive");
+  if (!thickness || !vp || !vs || !rho || !row_offsets || !periods || !period_offsets || !modes || !bounds ||
+      !velocity || !flags || !levels || !counts)
+    return fail(2, "null input or output pointer");
+  const bool want_group = s->cfg.group_velocity != 0;
+  if (want_group && (!group || !group_flags)) return fail(2, "group outputs required");
+  std::vector<mdr::PrepRow> rows(static_cast<size_t>(row_offsets[B]));
+  std::vector<mdr::ModelInfo> models(B);
+  std::vector<mdr::TraceInfo> traces(static_cast<size_t>(period_offsets[B]));
+  const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
+  long long out0 = 0;
+  int kmax = 0;
+  for (int m = 0; m < B; ++m) {
+    const long long r0 = row_offsets[m], r1 = row_offsets[m + 1];
+    const long long p0 = period_offsets[m], p1 = period_offsets[m + 1];
+    const int L = static_cast<int>(r1 - r0), T = static_cast<int>(p1 - p0);
+    if (L < 1 || L > mdr::kMaxRows || T < 0 || modes[m] < 1 || modes[m] > 256)
+      return fail(2, "invalid ragged geometry at model " + std::to_string(m));
+    for (long long j = r0; j < r1; ++j) rows[j] = mdr::make_prep_row(thickness[j], vp[j], vs[j], rho[j]);
+    double slow;
+    models[m] = mdr::make_model_info(thickness + r0, vp + r0, vs + r0, L, bounds[2 * m], bounds[2 * m + 1], wave,
+                                     modes[m], T, r0, out0, p0, &slow);
+    for (int f = 0; f < T; ++f) traces[p0 + f] = mdr::make_trace(m, f, periods[p0 + f], slow, pp);
+    out0 += static_cast<long long>(modes[m]) * T;
+    kmax = std::max(kmax, modes[m] + 2);
+  }
+  const long long ntr = period_offsets[B];
+  const size_t ncell = static_cast<size_t>(out0);
+  if (int e = s->rows.ensure(rows.size() * sizeof(mdr::PrepRow))) return e;
+  if (int e = s->models.ensure(models.size() * sizeof(mdr::ModelInfo))) return e;
+  if (int e = s->traces.ensure(traces.size() * sizeof(mdr::TraceInfo))) return e;
+  if (int e = s->out_v.ensure(ncell * sizeof(double))) return e;
+  if (int e = s->out_flags.ensure(static_cast<size_t>(ntr) * sizeof(uint32_t))) return e;
+  if (int e = s->out_levels.ensure(static_cast<size_t>(ntr) * sizeof(int32_t))) return e;
+  if (int e = s->out_counts.ensure(static_cast<size_t>(ntr) * sizeof(int32_t))) return e;
+  if (want_group) {
+    if (int e = s->out_group.ensure(ncell * sizeof(double))) return e;
+    if (int e = s->out_gflags.ensure(ncell * sizeof(uint32_t))) return e;
+  }
+  hipStream_t st = s->stream;
+  MD_HIP(hipEventRecord(s->ev_begin, st));
+  MD_HIP(hipMemcpyAsync(s->rows.p, rows.data(), rows.size() * sizeof(mdr::PrepRow), hipMemcpyHostToDevice, st));
+  MD_HIP(hipMemcpyAsync(s->models.p, models.data(), models.size() * sizeof(mdr::ModelInfo), hipMemcpyHostToDevice,
+                        st));
+  MD_HIP(hipMemcpyAsync(s->traces.p, traces.data(), traces.size() * sizeof(mdr::TraceInfo), hipMemcpyHostToDevice,
+                        st));
+  if (int e = run_pipeline(s, ntr, kmax, wave, c_tol_rel, s->out_v.as<double>(), nullptr,
+                           s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(), s->out_counts.as<int32_t>(),
+                           want_group ? s->out_group.as<double>() : nullptr,
+                           want_group ? s->out_gflags.as<uint32_t>() : nullptr))
+    return e;
+  MD_HIP(hipEventRecord(s->ev_end, st));
+  MD_HIP(hipMemcpyAsync(velocity, s->out_v.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
+  MD_HIP(hipMemcpyAsync(flags, s->out_flags.p, static_cast<size_t>(ntr) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        st));
+  MD_HIP(hipMemcpyAsync(levels, s->out_levels.p, static_cast<size_t>(ntr) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                        st));
+  MD_HIP(hipMemcpyAsync(counts, s->out_counts.p, static_cast<size_t>(ntr) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                        st));
+  if (want_group) {
+    MD_HIP(hipMemcpyAsync(group, s->out_group.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
+    MD_HIP(hipMemcpyAsync(group_flags, s->out_gflags.p, ncell * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  }
+  return md_rocm_synchronize(s);
+}
+
+}  // extern "C"
