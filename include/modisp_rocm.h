@@ -100,9 +100,11 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
 /* Wait for the solver's stream; returns the first asynchronous error. */
 int md_rocm_synchronize(md_rocm_solver *s);
 
-/* Kernel time of the last run in milliseconds (HIP events on the solver's
- * stream), per stage: 0 total, 1 scan, 2 recheck, 3 trough, 4 refine, 5 group.
- * Returns the number of stages written. */
+/* Device time of the last run in milliseconds, from HIP events recorded on the
+ * solver's stream (summed over trace chunks): 0 whole call, 1 grid scan,
+ * 2 FP64 rescan of overflowed FP32 traces, 3 FP64 recheck + trough probes,
+ * 4 bracket assembly, 5 refinement + output, 6 group velocity.
+ * Returns the number of values written (at most 7). */
 int md_rocm_last_timing(md_rocm_solver *s, double *ms, int n);
 
 /* Automatic search window [B, 2] (reference _cpu.bounds, cpp/cpu/solver.cpp:781-790):
@@ -110,6 +112,15 @@ int md_rocm_last_timing(md_rocm_solver *s, double *ms, int n);
  * auto_low == 0 (then `low`), upper = min(high or Vs_halfspace, nextafter(Vs_halfspace, 0)). */
 int md_rocm_bounds(const double *thickness, const double *vp, const double *vs, const double *rho, int B, int L,
                    int wave, int auto_low, double low, int auto_high, double high, double *out);
+
+/* Secular values on the scan grid of each (model, period), computed on the
+ * device with this backend's FP32 (fp32 = 1) or FP64 evaluator: host inputs
+ * [B, L], periods [nf], bounds [B, 2]; host output [B, nf, nc] float64 with
+ * node i at lo + i (hi - lo) / (nc - 1). Diagnostic counterpart of the
+ * reference's probe_grid / md_dunkin_grid[_f32] (cpp/cpu/solver.cpp:811-816,
+ * cpp/cpu/dunkin_ref.cpp:41-83). */
+int md_rocm_probe_grid(const double *thickness, const double *vp, const double *vs, const double *rho, int B, int L,
+                       const double *periods, int nf, const double *bounds, int nc, int wave, int fp32, double *out);
 
 /* Device helpers (reference cpp/cuda/qe.cu:1140-1142, bindings.cpp:354-357). */
 int md_rocm_available(int device); /* 1 if the device exists and is gfx950-capable */

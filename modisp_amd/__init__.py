@@ -91,6 +91,19 @@ def bounds(thickness, vp, vs, rho, wave=0, low=None, high=None):
     return out
 
 
+def probe_grid(thickness, vp, vs, rho, periods, bounds, nc=512, wave=0, fp32=True):
+    """F on each (model, period) scan grid with the device evaluators: [B, nf, nc]."""
+    h, a, b, r = (_rows(x, n) for x, n in zip((thickness, vp, vs, rho), ("thickness", "vp", "vs", "rho")))
+    B, L = h.shape
+    per = np.ascontiguousarray(periods, dtype=np.float64).ravel()
+    bnd = np.ascontiguousarray(bounds, dtype=np.float64).reshape(B, 2)
+    out = np.empty((B, per.size, int(nc)))
+    _native.check(_native.lib().md_rocm_probe_grid(_native.ptr(h), _native.ptr(a), _native.ptr(b), _native.ptr(r), B,
+                                                   L, _native.ptr(per), per.size, _native.ptr(bnd), int(nc),
+                                                   int(wave), int(bool(fp32)), _native.ptr(out)))
+    return out
+
+
 class QeSolver:
     """Persistent native engine; the constructor and run() mirror the reference
     QeSolver (cpp/cuda/qe.cu:1166 signature, run at qe.cu:1060-1118).
@@ -224,9 +237,12 @@ class QeSolver:
     def synchronize(self):
         _native.check(_native.lib().md_rocm_synchronize(self.handle))
 
+    STAGES = ("total", "scan", "rescan", "recheck", "assemble", "refine", "group")
+
     def last_timing(self):
-        ms = np.zeros(6)
-        _native.lib().md_rocm_last_timing(self.handle, _native.ptr(ms), 6)
+        """Device milliseconds of the last run per stage (see STAGES)."""
+        ms = np.zeros(len(self.STAGES))
+        _native.lib().md_rocm_last_timing(self.handle, _native.ptr(ms), len(self.STAGES))
         return ms
 
     def close(self):

@@ -38,7 +38,7 @@ _lib = None
 EXPORTS = ("md_rocm_default_config", "md_rocm_create", "md_rocm_free", "md_rocm_run", "md_rocm_run_device",
            "md_rocm_run_ragged", "md_rocm_synchronize", "md_rocm_last_timing", "md_rocm_bounds",
            "md_rocm_available", "md_rocm_set_device", "md_rocm_memory_info", "md_rocm_device_name",
-           "md_rocm_last_error")
+           "md_rocm_last_error", "md_rocm_probe_grid")
 
 
 def lib():
@@ -84,6 +84,9 @@ def lib():
     L.md_rocm_memory_info.restype = ctypes.c_int
     L.md_rocm_device_name.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
     L.md_rocm_device_name.restype = ctypes.c_int
+    L.md_rocm_probe_grid.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    L.md_rocm_probe_grid.restype = ctypes.c_int
     L.md_rocm_last_error.argtypes = []
     L.md_rocm_last_error.restype = ctypes.c_char_p
     _lib = L

@@ -15,6 +15,8 @@
 //                      inside the branch-point and low-velocity guards) and
 //                      same-sign troughs; uncertain nodes and troughs are
 //                      appended to device work queues.
+//   rescan_kernel      FP64 rescan (persistent) of FP32 traces whose event
+//                      list overflowed: no node of an FP64 grid is uncertain.
 //   recheck_kernel     FP64 re-evaluation of every queued uncertain node,
 //                      one lane per node (dense across traces).
 //   trough_kernel      FP64 golden-section probe of each queued trough for a
@@ -122,16 +124,22 @@ __device__ __forceinline__ Count4 block_scan4(Count4 v, Count4 *wave_tot, Count4
 // Scan: FP32 (or FP64) uniform grid, classification and event compaction.
 // ---------------------------------------------------------------------------
 template <class T, int S>
-__global__ __launch_bounds__(kBlock) void scan_kernel(ScanArgs a) {
-  constexpr int NC = kBlock * S;
-  __shared__ Layer<T> lay[kMaxRows];
-  __shared__ float fval[NC];
-  __shared__ unsigned char unc[NC];
-  __shared__ Count4 wave_tot[kBlock / 64];
-  __shared__ int cut_node;
+struct ScanShared {
+  Layer<T> lay[kMaxRows];
+  float fval[kBlock * S];
+  unsigned char unc[kBlock * S];
+  Count4 wave_tot[kBlock / 64];
+  int cut_node, q_unc, q_tr;
+};
 
-  const int t = blockIdx.x + a.trace_begin;
-  if (t >= a.trace_end) return;
+// Scans trace t with the whole block. Every return is block-uniform.
+template <class T, int S>
+__device__ void scan_trace(const PipeArgs &a, int t, ScanShared<T, S> &sh) {
+  constexpr int NC = kBlock * S;
+  Layer<T> *lay = sh.lay;
+  float *fval = sh.fval;
+  unsigned char *unc = sh.unc;
+  Count4 *wave_tot = sh.wave_tot;
   const TraceInfo ti = a.traces[t];
   const ModelInfo mi = a.models[ti.model];
   const int L = mi.L, top = mi.top, K = mi.K;
@@ -154,27 +162,28 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(ScanArgs a) {
   const double lo = mi.lo, dc = (mi.hi - mi.lo) / double(NC - 1);
   const bool fp32 = sizeof(T) == 4;
   const float tau = fp32 ? ti.tau : 0.f;
-  const double hi_guard = mi.cutoff_guard, lo_guard = fp32 ? mi.stable_low : 0.;
+  const double hi_guard = mi.cutoff_guard;
+  // Far below the largest shear speed the FP32 compound matrix cancels
+  // (reference limit: max Vs / 16); such windows go straight to the FP64 grid.
+  if (fp32 && lo < mi.stable_low) {
+    if (tid == 0) {
+      ts->nev = 0;
+      ts->flags = kTraceOverflow;
+      ts->status = 0;
+      a.overflow_list[atomicAdd(a.overflow_count, 1)] = t - a.trace_begin;
+    }
+    return;
+  }
   // Evaluate: sample i = s*kBlock + tid (consecutive lanes, consecutive samples).
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     const int i = s * kBlock + tid;
     const double c = lo + dc * double(i);
-    float fv;
-    bool u;
-    if (c < lo_guard) {
-      // Far below every shear speed the FP32 compound matrix cancels; such
-      // samples are contiguous lanes, so whole waves take the FP64 path here.
-      fv = float(secular_rows(rows, L, top, a.wave, omega, c));
-      u = false;
-    } else {
-      const T k = T(omega / c);
-      const T f = a.wave == 0 ? rayleigh_F<T>(lay, L, top, k) : love_F<T>(lay, L, top, k);
-      fv = float(f);
-      u = fp32 && (!(fabsf(fv) >= tau) || c >= hi_guard);
-    }
+    const T k = T(omega / c);
+    const T f = a.wave == 0 ? rayleigh_F<T>(lay, L, top, k) : love_F<T>(lay, L, top, k);
+    const float fv = float(f);
     fval[i] = fv;
-    unc[i] = u ? 1 : 0;
+    unc[i] = (fp32 && (!(fabsf(fv) >= tau) || c >= hi_guard)) ? 1 : 0;
   }
   __syncthreads();
 
@@ -207,16 +216,16 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(ScanArgs a) {
   const Count4 incl = block_scan4(cnt, wave_tot, &total);
   const Count4 excl{incl.ev - cnt.ev, incl.br - cnt.br, incl.un - cnt.un, incl.tr - cnt.tr};
   // Cut after the K-th certain bracket (the first K roots lie at or before it).
-  if (tid == 0) cut_node = NC;
+  if (tid == 0) sh.cut_node = NC;
   __syncthreads();
   if (excl.br < K && incl.br >= K) {
     int b = excl.br;
 #pragma unroll
     for (int s = 0; s < S; ++s)
-      if (kind[s] == kBracket && ++b == K) cut_node = tid * S + s;
+      if (kind[s] == kBracket && ++b == K) sh.cut_node = tid * S + s;
   }
   __syncthreads();
-  const int cut = cut_node;
+  const int cut = sh.cut_node;
   // Events at or before the cut, and how many of them are uncertain / troughs.
   Count4 kept{0, 0, 0, 0};
 #pragma unroll
@@ -229,18 +238,21 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(ScanArgs a) {
   }
   Count4 ktot;
   const Count4 kin = block_scan4(kept, wave_tot, &ktot);
-  __shared__ int q_unc, q_tr;
   const bool overflow = ktot.ev > kEventCap;
   if (tid == 0) {
-    q_unc = (!overflow && ktot.un) ? atomicAdd(a.unc_count, ktot.un) : 0;
-    q_tr = (!overflow && ktot.tr) ? atomicAdd(a.trough_count, ktot.tr) : 0;
+    sh.q_unc = (!overflow && ktot.un) ? atomicAdd(a.unc_count, ktot.un) : 0;
+    sh.q_tr = (!overflow && ktot.tr) ? atomicAdd(a.trough_count, ktot.tr) : 0;
     ts->nev = overflow ? 0 : ktot.ev;
     ts->flags = overflow ? kTraceOverflow : 0u;
     ts->status = 0;
     ts->certain = total.br;
+    // An FP32 trace whose events overflow is rescanned on the FP64 grid,
+    // which has no uncertain nodes.
+    if (overflow && fp32) a.overflow_list[atomicAdd(a.overflow_count, 1)] = t - a.trace_begin;
   }
   __syncthreads();
   if (overflow) return;
+  const int q_unc = sh.q_unc, q_tr = sh.q_tr;
   int e = kin.ev - kept.ev, u = kin.un - kept.un, r = kin.tr - kept.tr;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
@@ -262,6 +274,24 @@ __global__ __launch_bounds__(kBlock) void scan_kernel(ScanArgs a) {
     u += kind[s] == kUncertain;
     r += kind[s] == kTrough;
     ++e;
+  }
+}
+
+template <class T, int S>
+__global__ __launch_bounds__(kBlock) void scan_kernel(PipeArgs a) {
+  __shared__ ScanShared<T, S> sh;
+  const int t = blockIdx.x + a.trace_begin;
+  if (t < a.trace_end) scan_trace<T, S>(a, t, sh);
+}
+
+// FP64 rescan of the traces whose FP32 event list overflowed (persistent).
+template <int S>
+__global__ __launch_bounds__(kBlock) void rescan_kernel(PipeArgs a) {
+  __shared__ ScanShared<double, S> sh;
+  const int n = *a.overflow_count;
+  for (int q = blockIdx.x; q < n; q += gridDim.x) {
+    scan_trace<double, S>(a, a.overflow_list[q] + a.trace_begin, sh);
+    __syncthreads();
   }
 }
 
@@ -559,7 +589,7 @@ __global__ void reset_kernel(int *counters, int n) {
 namespace mdr {
 
 template <class T>
-static hipError_t launch_scan_t(const ScanArgs &a, int nc, hipStream_t s) {
+static hipError_t launch_scan_t(const PipeArgs &a, int nc, hipStream_t s) {
   const int n = a.trace_end - a.trace_begin;
   if (n <= 0) return hipSuccess;
   switch (nc) {
@@ -571,20 +601,41 @@ static hipError_t launch_scan_t(const ScanArgs &a, int nc, hipStream_t s) {
   return hipGetLastError();
 }
 
+static hipError_t launch_rescan(const PipeArgs &a, int nc, int blocks, hipStream_t s) {
+  switch (nc) {
+    case 256: hipLaunchKernelGGL((rescan_kernel<1>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case 512: hipLaunchKernelGGL((rescan_kernel<2>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    case 1024: hipLaunchKernelGGL((rescan_kernel<4>), dim3(blocks), dim3(kBlock), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bool group, int persistent_blocks,
-                           hipStream_t s) {
+                           hipStream_t s, hipEvent_t *marks) {
   const int n = sa.trace_end - sa.trace_begin;
   if (n <= 0) return hipSuccess;
+  auto mark = [&](int i) {
+    if (marks) (void)hipEventRecord(marks[i], s);
+  };
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, sa.counters, 4);
+  mark(0);
   hipError_t err = fp32 ? launch_scan_t<float>(sa, pa.nc, s) : launch_scan_t<double>(sa, pa.nc, s);
   if (err != hipSuccess) return err;
+  mark(1);
   const int pb = persistent_blocks;
+  if (fp32 && (err = launch_rescan(pa, pa.nc, pb, s)) != hipSuccess) return err;
+  mark(2);
   hipLaunchKernelGGL(recheck_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   hipLaunchKernelGGL(trough_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+  mark(3);
   hipLaunchKernelGGL(assemble_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
+  mark(4);
   hipLaunchKernelGGL(refine_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   hipLaunchKernelGGL(finalize_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
+  mark(5);
   if (group) hipLaunchKernelGGL(group_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+  mark(6);
   return hipGetLastError();
 }
 
@@ -625,6 +676,46 @@ hipError_t launch_prep(const double *h, const double *vp, const double *vs, cons
   const long long n = static_cast<long long>(B) * T;
   hipLaunchKernelGGL(prep_traces_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, periods,
                      row_periods, B, T, models, p, traces);
+  return hipGetLastError();
+}
+
+}  // namespace mdr
+
+// ---------------------------------------------------------------------------
+// Diagnostic grid probe: F on the scan grid with the scan's own evaluators.
+// ---------------------------------------------------------------------------
+namespace mdr {
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void probe_kernel(const TraceInfo *traces, const ModelInfo *models,
+                                                       const PrepRow *rows_all, int ntraces, int nc, int wave,
+                                                       double *out) {
+  __shared__ Layer<T> lay[kMaxRows];
+  const int t = blockIdx.x;
+  if (t >= ntraces) return;
+  const TraceInfo ti = traces[t];
+  const ModelInfo mi = models[ti.model];
+  const PrepRow *rows = rows_all + mi.row0;
+  const double omega = ti.omega, iw2 = 1. / (omega * omega);
+  for (int j = threadIdx.x; j < mi.L; j += kBlock) lay[j] = make_layer<T>(rows[j], omega, iw2, wave);
+  __syncthreads();
+  const double dc = (mi.hi - mi.lo) / double(nc - 1);
+  for (int i = threadIdx.x; i < nc; i += kBlock) {
+    const double c = mi.lo + dc * double(i);
+    const T k = T(omega / c);
+    const T f = wave == 0 ? rayleigh_F<T>(lay, mi.L, mi.top, k) : love_F<T>(lay, mi.L, mi.top, k);
+    out[static_cast<size_t>(t) * nc + i] = double(f);
+  }
+}
+
+hipError_t launch_probe(const TraceInfo *traces, const ModelInfo *models, const PrepRow *rows, int ntraces, int nc,
+                        int wave, int fp32, double *out, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL(probe_kernel<float>, dim3(ntraces), dim3(kBlock), 0, s, traces, models, rows, ntraces, nc,
+                       wave, out);
+  else
+    hipLaunchKernelGGL(probe_kernel<double>, dim3(ntraces), dim3(kBlock), 0, s, traces, models, rows, ntraces, nc,
+                       wave, out);
   return hipGetLastError();
 }
 

@@ -103,9 +103,11 @@ struct PipeArgs {
   double *roots;
   uint32_t *unc_queue;
   uint32_t *trough_queue;
-  int *counters;  // [0] uncertain nodes, [1] troughs
+  int *counters;  // [0] uncertain nodes, [1] troughs, [2] overflowed FP32 traces
   int *unc_count;
   int *trough_count;
+  int *overflow_count;
+  int *overflow_list;  // chunk-local trace indices to rescan in FP64
   int queue_cap;
   int trace_begin, trace_end;
   int nc;
@@ -124,8 +126,11 @@ struct PipeArgs {
 };
 using ScanArgs = PipeArgs;
 
+// marks (optional): 7 events recorded before the scan and after the scan,
+// FP64 rescan, recheck/trough, assemble, refine/finalize and group stages.
+constexpr int kStageMarks = 7;
 hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bool group, int persistent_blocks,
-                           hipStream_t s);
+                           hipStream_t s, hipEvent_t *marks);
 
 }  // namespace mdr
 
@@ -212,4 +217,9 @@ namespace mdr {
 hipError_t launch_prep(const double *h, const double *vp, const double *vs, const double *rho, const double *bounds,
                        const double *periods, int row_periods, int B, int L, int T, int modes, const PrepParams &p,
                        PrepRow *rows, ModelInfo *models, TraceInfo *traces, hipStream_t s);
+}  // namespace mdr
+
+namespace mdr {
+hipError_t launch_probe(const TraceInfo *traces, const ModelInfo *models, const PrepRow *rows, int ntraces, int nc,
+                        int wave, int fp32, double *out, hipStream_t s);
 }  // namespace mdr

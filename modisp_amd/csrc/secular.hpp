@@ -124,16 +124,39 @@ template <class T>
 struct Wave {
   T c, w, x, att;
 };
+// r and 1/r from r^2: one hardware reciprocal square root for float, the
+// correctly rounded sqrt and division for double.
+MDR_HD void root_pair(float r2, float *r, float *ir) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float q = __builtin_amdgcn_rsqf(r2);
+  *ir = q;
+  *r = r2 * q;
+#else
+  *r = std::sqrt(r2);
+  *ir = 1.f / *r;
+#endif
+}
+MDR_HD void root_pair(double r2, double *r, double *ir) {
+  *r = sqrt(r2);
+  *ir = 1. / *r;
+}
+template <class T>
+MDR_HD void root_pair(T r2, T *r, T *ir) {
+  *r = m_sqrt(r2);
+  *ir = m_rcp(*r);
+}
+
 template <class T>
 MDR_HD Wave<T> wave_factor(T k, T kx, T h) {
   Wave<T> o;
-  const T r = m_sqrt((k + kx) * m_fabs(k - kx));
+  T r, ir;
+  root_pair((k + kx) * m_fabs(k - kx), &r, &ir);
   const T p = r * h;
   if (k < kx) {  // propagating: cos p, sin p / r, -r sin p
     T s, c;
     m_sincos(p, &s, &c);
     o.c = c;
-    o.w = s * m_rcp(r);
+    o.w = s * ir;
     o.x = -r * s;
     o.att = T(1);
   } else if (k == kx) {  // c equals the body-wave speed: the r -> 0 limit
@@ -146,7 +169,7 @@ MDR_HD Wave<T> wave_factor(T k, T kx, T h) {
     const T f = e * e;
     o.c = (T(1) + f) * T(0.5);
     const T s = (T(1) - f) * T(0.5);
-    o.w = s * m_rcp(r);
+    o.w = s * ir;
     o.x = r * s;
     o.att = e;
   }

@@ -70,16 +70,20 @@ struct md_rocm_solver {
   int cus = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats;
+  DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats, overflow;
   DevBuf in_h, in_vp, in_vs, in_rho, in_per, in_bounds;
   DevBuf out_v, out_flags, out_levels, out_counts, out_group, out_gflags;
-  double last_ms[6] = {0, 0, 0, 0, 0, 0};
+  double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<hipEvent_t> marks;  // kStageMarks per chunk of the last run
+  int marked_chunks = 0;
 
   ~md_rocm_solver() {
     for (DevBuf *b : {&rows, &models, &traces, &events, &scan, &brackets, &roots, &unc_q, &tr_q, &counters, &stats,
+                      &overflow,
                       &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &out_v, &out_flags, &out_levels,
                       &out_counts, &out_group, &out_gflags})
       b->release();
+    for (hipEvent_t e : marks) (void)hipEventDestroy(e);
     if (ev_begin) (void)hipEventDestroy(ev_begin);
     if (ev_end) (void)hipEventDestroy(ev_end);
     if (stream) (void)hipStreamDestroy(stream);
@@ -107,6 +111,7 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, doubl
   if (int e = s->unc_q.ensure(qcap * sizeof(uint32_t))) return e;
   if (int e = s->tr_q.ensure(qcap * sizeof(uint32_t))) return e;
   if (int e = s->counters.ensure(4 * sizeof(int))) return e;
+  if (int e = s->overflow.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   mdr::PipeArgs a{};
   a.traces = s->traces.as<mdr::TraceInfo>();
   a.models = s->models.as<mdr::ModelInfo>();
@@ -120,6 +125,8 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, doubl
   a.counters = s->counters.as<int>();
   a.unc_count = a.counters;
   a.trough_count = a.counters + 1;
+  a.overflow_count = a.counters + 2;
+  a.overflow_list = s->overflow.as<int>();
   a.queue_cap = static_cast<int>(qcap);
   a.nc = s->cfg.nc;
   a.kmax = kmax;
@@ -135,10 +142,19 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, doubl
   a.group = group;
   a.group_flags = gflags;
   const int persistent = s->cus * 8;
-  for (long long begin = 0; begin < ntraces; begin += chunk) {
+  const int nchunks = static_cast<int>((ntraces + chunk - 1) / chunk);
+  while (static_cast<int>(s->marks.size()) < nchunks * mdr::kStageMarks) {
+    hipEvent_t e;
+    MD_HIP(hipEventCreate(&e));
+    s->marks.push_back(e);
+  }
+  s->marked_chunks = nchunks;
+  int c = 0;
+  for (long long begin = 0; begin < ntraces; begin += chunk, ++c) {
     a.trace_begin = static_cast<int>(begin);
     a.trace_end = static_cast<int>(std::min<long long>(ntraces, begin + chunk));
-    const hipError_t err = mdr::launch_pipeline(a, a, a.fp32 != 0, group != nullptr, persistent, s->stream);
+    const hipError_t err = mdr::launch_pipeline(a, a, a.fp32 != 0, group != nullptr, persistent, s->stream,
+                                                s->marks.data() + c * mdr::kStageMarks);
     if (err != hipSuccess) return fail(1, std::string("kernel launch: ") + hipGetErrorString(err));
   }
   return 0;
@@ -237,13 +253,19 @@ int md_rocm_synchronize(md_rocm_solver *s) {
   if (!s) return fail(2, "null solver");
   MD_HIP(hipStreamSynchronize(s->stream));
   float ms = 0.f;
+  for (double &x : s->last_ms) x = 0.;
   if (hipEventElapsedTime(&ms, s->ev_begin, s->ev_end) == hipSuccess) s->last_ms[0] = ms;
+  for (int c = 0; c < s->marked_chunks; ++c) {
+    hipEvent_t *m = s->marks.data() + c * mdr::kStageMarks;
+    for (int k = 0; k + 1 < mdr::kStageMarks; ++k)
+      if (hipEventElapsedTime(&ms, m[k], m[k + 1]) == hipSuccess) s->last_ms[1 + k] += ms;
+  }
   return 0;
 }
 
 int md_rocm_last_timing(md_rocm_solver *s, double *ms, int n) {
   if (!s || !ms) return 0;
-  const int k = std::min(n, 6);
+  const int k = std::min(n, 7);
   for (int i = 0; i < k; ++i) ms[i] = s->last_ms[i];
   return k;
 }
@@ -420,3 +442,50 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
 }
 
 }  // extern "C"
+
+extern "C" int md_rocm_probe_grid(const double *thickness, const double *vp, const double *vs, const double *rho,
+                                  int B, int L, const double *periods, int nf, const double *bounds, int nc, int wave,
+                                  int fp32, double *out) {
+  if (!thickness || !vp || !vs || !rho || !periods || !bounds || !out) return fail(2, "null pointer");
+  if (B < 1 || L < 1 || L > mdr::kMaxRows || nf < 1 || nc < 2 || (wave != 0 && wave != 1))
+    return fail(2, "invalid probe geometry");
+  const size_t nrow = static_cast<size_t>(B) * L, ntr = static_cast<size_t>(B) * nf;
+  DevBuf h, a, b, r, per, bnd, rows, models, traces, res;
+  for (DevBuf *x : {&h, &a, &b, &r})
+    if (int e = x->ensure(nrow * sizeof(double))) return e;
+  if (int e = per.ensure(nf * sizeof(double))) return e;
+  if (int e = bnd.ensure(static_cast<size_t>(B) * 2 * sizeof(double))) return e;
+  if (int e = rows.ensure(nrow * sizeof(mdr::PrepRow))) return e;
+  if (int e = models.ensure(static_cast<size_t>(B) * sizeof(mdr::ModelInfo))) return e;
+  if (int e = traces.ensure(ntr * sizeof(mdr::TraceInfo))) return e;
+  if (int e = res.ensure(ntr * nc * sizeof(double))) return e;
+  auto cleanup = [&] {
+    for (DevBuf *x : {&h, &a, &b, &r, &per, &bnd, &rows, &models, &traces, &res}) x->release();
+  };
+  hipStream_t st = nullptr;
+  int code = 0;
+  do {
+    if (hipMemcpy(h.p, thickness, nrow * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(a.p, vp, nrow * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b.p, vs, nrow * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(r.p, rho, nrow * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(per.p, periods, nf * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(bnd.p, bounds, static_cast<size_t>(B) * 16, hipMemcpyHostToDevice) != hipSuccess) {
+      code = fail(1, "probe upload failed");
+      break;
+    }
+    const mdr::PrepParams pp{wave, 0., 0.};
+    if (mdr::launch_prep(h.as<double>(), a.as<double>(), b.as<double>(), r.as<double>(), bnd.as<double>(),
+                         per.as<double>(), 0, B, L, nf, 1, pp, rows.as<mdr::PrepRow>(), models.as<mdr::ModelInfo>(),
+                         traces.as<mdr::TraceInfo>(), st) != hipSuccess ||
+        mdr::launch_probe(traces.as<mdr::TraceInfo>(), models.as<mdr::ModelInfo>(), rows.as<mdr::PrepRow>(),
+                          static_cast<int>(ntr), nc, wave, fp32, res.as<double>(), st) != hipSuccess) {
+      code = fail(1, "probe launch failed");
+      break;
+    }
+    if (hipMemcpy(out, res.p, ntr * nc * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      code = fail(1, "probe download failed");
+  } while (false);
+  cleanup();
+  return code;
+}

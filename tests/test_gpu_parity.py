@@ -1,0 +1,247 @@
+"""GPU parity of the gfx950 root pipeline against the oracle.
+
+Bar: identical root counts and validity masks; phase velocities within 1e-9
+relative; analytic group velocities within 1e-9 relative of the oracle's
+analytic implicit derivative. Every call goes through the C ABI
+(modisp_amd.QeSolver -> libmodisp_rocm.so). Sizes are chosen so the oracle
+finishes in seconds; full-size runs are checked through size-independent
+properties (FP32 path == FP64 path, ascending roots inside the window).
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).parent / "golden"
+MODEL = ("thickness", "vp", "vs", "rho")
+RTOL = 1e-9
+
+
+@pytest.fixture(scope="module")
+def md():
+    import modisp_amd
+    if not modisp_amd.available(0):
+        pytest.fail("no gfx950 device visible to the gpu tests")
+    modisp_amd.set_device(0)
+    return modisp_amd
+
+
+def gpu_solve(md, h, vp, vs, rho, periods, bounds, nmodes, wave, scan="float32", group=True, nc=512):
+    h, vp, vs, rho = (np.atleast_2d(np.asarray(x, dtype=np.float64)) for x in (h, vp, vs, rho))
+    B, L = h.shape
+    per = np.asarray(periods, dtype=np.float64)
+    nf = per.shape[-1]
+    with md.QeSolver(B, L, nf, nmodes, group_velocity=group, scan=scan, nc=nc) as s:
+        vel, flags, st, lev, cnt, ex = s.run(h, vp, vs, rho, per, bounds, 1e-12, 1., wave, False)
+        return dict(phase_velocity=vel, trace_flags=flags, search_level=lev, root_count=cnt,
+                    group_velocity=ex["group_velocity"], group_velocity_flags=ex["group_velocity_flags"],
+                    timing=s.last_timing())
+
+
+def assert_parity(got, ref, group=True, label=""):
+    np.testing.assert_array_equal(got["root_count"], ref["root_count"], err_msg=f"{label} root counts")
+    gv, rv = got["phase_velocity"], ref["phase_velocity"]
+    np.testing.assert_array_equal(np.isfinite(gv), np.isfinite(rv), err_msg=f"{label} validity mask")
+    ok = np.isfinite(rv)
+    if ok.any():
+        rel = np.abs(gv[ok] - rv[ok]) / rv[ok]
+        assert rel.max() <= RTOL, f"{label} phase rel {rel.max():.3e}"
+    if group:
+        gg, rg = got["group_velocity"], ref["group_velocity"]
+        np.testing.assert_array_equal(np.isfinite(gg), np.isfinite(rg), err_msg=f"{label} group mask")
+        gok = np.isfinite(rg)
+        if gok.any():
+            grel = np.abs(gg[gok] - rg[gok]) / np.abs(rg[gok])
+            assert grel.max() <= RTOL, f"{label} group rel {grel.max():.3e}"
+        np.testing.assert_array_equal(got["group_velocity_flags"], ref["group_velocity_flags"],
+                                      err_msg=f"{label} group flags")
+
+
+# --- the precision workload (FP32 scan + FP64 refine) --------------------------------------------
+@pytest.mark.parametrize("wave", [0, 1], ids=["rayleigh", "love"])
+@pytest.mark.parametrize("scan", ["float32", "float64"])
+def test_precision_workload_subset(md, wave, scan):
+    from modisp_amd import workloads
+    wl = workloads.precision_batch(B=24, seed=101)
+    win = workloads.scan_window(wl.vs)
+    got = gpu_solve(md, *(getattr(wl, k) for k in MODEL), wl.periods, win, 5, wave, scan)
+    ref = oracle.solve(*(getattr(wl, k) for k in MODEL), wl.periods, win, 5, wave, tol=1e-12, group=2)
+    assert_parity(got, ref, label=f"wave {wave} scan {scan}")
+
+
+@pytest.mark.parametrize("nc", [256, 1024])
+def test_other_grid_sizes(md, nc):
+    from modisp_amd import workloads
+    wl = workloads.precision_batch(B=8, seed=102)
+    win = workloads.scan_window(wl.vs)
+    per = wl.periods[::4]
+    got = gpu_solve(md, *(getattr(wl, k) for k in MODEL), per, win, 5, 0, "float32", nc=nc)
+    ref = oracle.solve(*(getattr(wl, k) for k in MODEL), per, win, 5, 0, tol=1e-12, group=2)
+    assert_parity(got, ref, label=f"nc {nc}")
+
+
+def test_plain_workload_subset_automatic_window(md):
+    """The CPU-runnable workload with the reference's automatic search window."""
+    from modisp_amd import workloads
+    wl = workloads.plain_batch(B=64, seed=0)
+    b = md.bounds(wl.thickness, wl.vp, wl.vs, wl.rho, 0)
+    got = gpu_solve(md, *(getattr(wl, k) for k in MODEL), wl.periods, b, 3, 0, "float32")
+    ref = oracle.solve(*(getattr(wl, k) for k in MODEL), wl.periods, b, 3, 0, automatic=True, tol=1e-12, group=2)
+    assert_parity(got, ref, label="plain automatic")
+
+
+def test_sweep_workload_subset(md):
+    """Random-walk models (low-velocity zones, Vp/Vs 1.65-2.0), Rayleigh and Love."""
+    from modisp_amd import workloads
+    wl = workloads.sweep_batch(B=12, seed=104)
+    per = wl.periods[::8]
+    for wave in (0, 1):
+        b = md.bounds(wl.thickness, wl.vp, wl.vs, wl.rho, wave)
+        win = np.stack([np.maximum(b[:, 0], 0.8 * wl.vs.min(axis=1)), b[:, 1]], axis=1)
+        got = gpu_solve(md, *(getattr(wl, k) for k in MODEL), per, win, 4, wave, "float32")
+        ref = oracle.solve(*(getattr(wl, k) for k in MODEL), per, win, 4, wave, tol=1e-12, group=2)
+        assert_parity(got, ref, label=f"sweep wave {wave}")
+
+
+def test_ragged_uneven_batch(md):
+    """Ragged models (3..40 finite layers, low-velocity zones, per-model
+    period and mode counts) through the ragged entry point."""
+    from modisp_amd import workloads
+    wl = workloads.uneven_batch(B=24, seed=105)
+    rows, row_off, per, per_off, modes, bounds = [], [0], [], [0], [], []
+    for i in range(wl.B):
+        n = wl.layers[i]
+        rows.append(np.stack([a[i, :n] for a in (wl.thickness, wl.vp, wl.vs, wl.rho)]))
+        row_off.append(row_off[-1] + n)
+        p = wl.periods[i][::3]
+        per.append(p)
+        per_off.append(per_off[-1] + p.size)
+        modes.append(int(wl.nmodes[i]))
+        bounds.append([0.8 * wl.vs[i, :n].min(), np.nextafter(wl.vs[i, n - 1], 0.)])
+    packed = np.concatenate(rows, axis=1)
+    with md.QeSolver(wl.B, int(max(wl.layers)), 1, 1, group_velocity=True) as s:
+        out = s.run_ragged(*packed, row_off, np.concatenate(per), per_off, modes, np.array(bounds), 1e-12, 0)
+    cell = 0
+    for i in range(wl.B):
+        n, T, M = wl.layers[i], per_off[i + 1] - per_off[i], modes[i]
+        ref = oracle.solve(*(packed[:, row_off[i]:row_off[i + 1]]), per[i], np.array(bounds[i])[None], M, 0,
+                           tol=1e-12, group=2)
+        got = dict(root_count=out["counts"][per_off[i]:per_off[i + 1]][None],
+                   phase_velocity=out["velocity"][cell:cell + M * T].reshape(1, M, T),
+                   group_velocity=out["group_velocity"][cell:cell + M * T].reshape(1, M, T),
+                   group_velocity_flags=out["group_velocity_flags"][cell:cell + M * T].reshape(1, M, T))
+        assert_parity(got, ref, label=f"ragged model {i} ({n} rows)")
+        cell += M * T
+
+
+# --- reference fixtures and closed forms --------------------------------------------------------
+@pytest.mark.parametrize("path", sorted((GOLDEN / "accuracy_high_cpu_linux").glob("*.npz")), ids=lambda p: p.stem)
+def test_reference_fixtures(md, path):
+    """Root counts and masks equal the reference's frozen accuracy="high"
+    results on every converged trace; velocities agree to the fixture's own
+    refinement tolerance (c_tol_rel 1e-6, legacy refinement)."""
+    f = np.load(path)
+    wave = int(str(f["wave"]) == "love")
+    cmin = float(f["cmin"]) if np.isfinite(f["cmin"]) else None
+    cmax = float(f["cmax"]) if np.isfinite(f["cmax"]) else None
+    h, vp, vs, rho = (np.atleast_2d(f[k]) for k in MODEL)
+    b = md.bounds(h, vp, vs, rho, wave, cmin, cmax)
+    np.testing.assert_array_equal(b, f["search_bounds"])
+    got = gpu_solve(md, h, vp, vs, rho, f["periods"], b, int(f["nmodes"]), wave, "float32", group=False)
+    converged = f["root_count"] >= 0
+    np.testing.assert_array_equal(got["root_count"][converged], f["root_count"][converged])
+    ref = f["phase_velocity"]
+    np.testing.assert_array_equal(np.isfinite(got["phase_velocity"])[np.broadcast_to(converged[:, None], ref.shape)],
+                                  f["mode_mask"][np.broadcast_to(converged[:, None], ref.shape)])
+    ok = np.isfinite(ref)
+    np.testing.assert_allclose(got["phase_velocity"][ok], ref[ok], rtol=2e-6, atol=3e-6)
+
+
+@pytest.mark.parametrize("path", sorted((GOLDEN / "witness").glob("*.npz")), ids=lambda p: p.stem)
+def test_independent_witnesses(md, path):
+    """Hard close-pair cases resolved by independent FP64 references."""
+    f = np.load(path)
+    cropped = path.name.startswith("B10000")
+    expected = (f["reference"] if cropped else f["reference_hi"]).ravel()
+    h, vp, vs, rho = (np.atleast_2d(f[k]) for k in MODEL)
+    b = md.bounds(h, vp, vs, rho, 0, .2 if cropped else None)
+    got = gpu_solve(md, h, vp, vs, rho, f["periods"], b, expected.size, 0, "float32", group=False)
+    np.testing.assert_allclose(got["phase_velocity"][0, :, 0], expected, atol=3e-6, rtol=2e-6)
+
+
+def test_two_layer_love_closed_form(md):
+    from test_oracle_pins import _two_layer_love, LOVE_MODEL
+    periods = np.array([.1, .5, 1., 2., 10.])
+    b = md.bounds(*(np.atleast_2d(x) for x in LOVE_MODEL), 1)
+    got = gpu_solve(md, *LOVE_MODEL, periods, b, 3, 1)
+    for i, p in enumerate(periods):
+        roots, group = _two_layer_love(p)
+        n = roots.size
+        np.testing.assert_allclose(got["phase_velocity"][0, :n, i], roots, rtol=1e-10)
+        np.testing.assert_allclose(got["group_velocity"][0, :n, i], group, rtol=1e-9)
+        assert np.isnan(got["phase_velocity"][0, n:, i]).all()
+
+
+def test_halfspace_rayleigh_and_empty_families(md):
+    from scipy.optimize import brentq
+    k = (3.5 / 6.) ** 2
+    x = brentq(lambda x: x ** 3 - 8 * x * x + (24 - 16 * k) * x - 16 * (1 - k), 0., 1., xtol=1e-15)
+    m = [np.array([[0.]]), np.array([[6.]]), np.array([[3.5]]), np.array([[2.7]])]
+    b = md.bounds(*m, 0)
+    got = gpu_solve(md, *m, [.1, 1., 10.], b, 1, 0)
+    np.testing.assert_allclose(got["phase_velocity"][0, 0], 3.5 * np.sqrt(x), rtol=1e-12)
+    np.testing.assert_allclose(got["group_velocity"][0, 0], got["phase_velocity"][0, 0], rtol=1e-9)
+    # Love on a half-space has no guided modes (reference level -3, NOT_FOUND).
+    love = gpu_solve(md, *m, [.1, 1.], md.bounds(*m, 1), 2, 1)
+    assert np.isnan(love["phase_velocity"]).all() and (love["root_count"] == 0).all()
+    assert (love["search_level"] == -3).all()
+    assert (love["group_velocity_flags"] == 1).all()
+    # An empty window (hi <= lo) reports level -4.
+    empty = gpu_solve(md, *m, [1.], np.array([[3., 2.]]), 1, 0)
+    assert (empty["search_level"] == -4).all() and (empty["root_count"] == 0).all()
+
+
+# --- full-size, size-independent properties ------------------------------------------------------
+@pytest.mark.parametrize("wave", [0, 1], ids=["rayleigh", "love"])
+def test_full_precision_workload_fp32_equals_fp64_path(md, wave):
+    from modisp_amd import workloads
+    wl = workloads.precision_batch()
+    win = workloads.scan_window(wl.vs)
+    a = gpu_solve(md, *(getattr(wl, k) for k in MODEL), wl.periods, win, 5, wave, "float32")
+    b = gpu_solve(md, *(getattr(wl, k) for k in MODEL), wl.periods, win, 5, wave, "float64")
+    assert (a["root_count"] >= 0).all()
+    assert_parity(a, b, label=f"fp32 vs fp64 wave {wave}")
+    v = a["phase_velocity"]
+    d = np.diff(v, axis=1)
+    assert np.all(d[np.isfinite(d)] > 0)
+    lo, hi = win[:, 0, None, None], win[:, 1, None, None]
+    fin = np.isfinite(v)
+    assert np.all((v >= lo)[fin]) and np.all((v <= hi)[fin])
+
+
+def test_fp32_grid_signs_outside_the_band(md):
+    """Wherever an FP32 grid value is outside the uncertainty band the pipeline
+    trusts its sign; it must equal the FP64 sign."""
+    from modisp_amd import workloads
+    wl = workloads.precision_batch(B=48, seed=106)
+    win = workloads.scan_window(wl.vs)
+    slow = np.sum(wl.thickness[:, :-1] / wl.vs[:, :-1], axis=1)
+    phi = (2 * np.pi / wl.periods)[None, :] * slow[:, None]
+    tau = np.maximum(1e-4, 5e-5 * (1 + phi))[:, :, None]
+    c = win[:, 0, None] + (win[:, 1] - win[:, 0])[:, None] * np.arange(512)[None, :] / 511.
+    guard = (c >= wl.vs[:, -1, None] * (1 - 1e-3))[:, None, :]
+    for wave in (0, 1):
+        f32 = md.probe_grid(wl.thickness, wl.vp, wl.vs, wl.rho, wl.periods, win, 512, wave, True)
+        f64 = md.probe_grid(wl.thickness, wl.vp, wl.vs, wl.rho, wl.periods, win, 512, wave, False)
+        trusted = (np.abs(f32) >= tau) & ~guard
+        assert np.array_equal(np.signbit(f32)[trusted], np.signbit(f64)[trusted])
+        # FP64 grid values agree in sign with the oracle's released evaluator.
+        for i in range(0, wl.B, 12):
+            for f in range(0, wl.periods.size, 25):
+                ref = oracle.secular(*(getattr(wl, k)[i] for k in MODEL), wl.periods[f], c[i], wave)
+                sure = np.abs(ref) > 1e-12
+                assert np.array_equal(np.signbit(ref)[sure], np.signbit(f64[i, f])[sure])
