@@ -40,10 +40,12 @@ typedef struct md_rocm_config {
   double group_rtol;       /* accepted for API parity; the analytic derivative is exact */
   int continuation;        /* anchor continuation across periods (0 = independent) */
   int anchor_stride;       /* anchors every <= stride periods, default 4 */
-  int nc;                  /* grid samples per trace: 256, 512 or 1024 (default 512) */
+  int nc;                  /* uniform grid: samples per trace, 3..65536 (default 512) */
   double tau_floor;        /* FP32 |F| uncertainty band floor (default 1e-4) */
   double tau_phase;        /* band growth per radian of vertical S phase (default 5e-5) */
   int chunk_traces;        /* traces per device chunk (0 = automatic) */
+  int grid;                /* 1: density-adaptive nodes (default), 0: uniform nc-node grid */
+  double rho;              /* adaptive grid: nodes per WKB mode (default 8) */
 } md_rocm_config;
 
 /* Defaults for every field. */

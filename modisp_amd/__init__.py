@@ -32,7 +32,7 @@ SUPPORTS_CANONICAL_LOWER = True
 SUPPORTS_ROW_PERIODS = True
 SUPPORTS_GROUP_VELOCITY = True
 SUPPORTS_CONTINUATION = False
-STAT_NAMES = ("traces", "fp64_rechecks", "trough_probes", "unresolved_traces")
+STAT_NAMES = ("traces", "fp64_rechecks", "pair_probes", "fp64_rescans")
 
 __all__ = ["load", "QeSolver", "available", "set_device", "memory_info", "device_name", "bounds",
            "BackendUnavailableError", "ABI_VERSION", "VERSION"]
@@ -111,15 +111,18 @@ class QeSolver:
     ``single_precision`` keeps the reference meaning of the returned dtype
     (float32 velocities). Extra keyword-only arguments select this backend's
     method: ``scan`` ("float32": FP32 grid scan with FP64 recheck and FP64
-    refinement, the default; "float64": the plain FP64 grid path), ``nc``
-    samples per trace and the FP32 uncertainty band (``tau_floor``,
-    ``tau_phase``).
+    refinement, the default; "float64": the plain FP64 grid path), ``grid``
+    ("adaptive": nodes uniform in the WKB mode count with ``rho`` nodes per
+    mode over a 64-point floor, the default; "uniform": ``nc`` equally spaced
+    nodes) and the FP32 uncertainty band (``tau_floor``, ``tau_phase``).
     """
 
     def __init__(self, B, L, nf, nmodes, accuracy=0, period_threshold=0., close_root_rtol=.005, shape=False,
                  sampling_rtol=.001, secular=0, refinement=1, love_phase=1, single_precision=False,
                  group_velocity=False, group_rtol=.001, continuation=False, anchor_stride=4, *, scan="float32",
-                 nc=512, tau_floor=None, tau_phase=None, chunk_traces=0):
+                 grid="adaptive", rho=8., nc=512, tau_floor=None, tau_phase=None, chunk_traces=0):
+        if grid not in ("adaptive", "uniform"):
+            raise ValueError("grid must be adaptive or uniform")
         if scan not in ("float32", "float64"):
             raise ValueError("scan must be float32 or float64")
         if continuation:
@@ -140,6 +143,8 @@ class QeSolver:
         if tau_phase is not None:
             cfg.tau_phase = float(tau_phase)
         cfg.chunk_traces = int(chunk_traces)
+        cfg.grid = int(grid == "adaptive")
+        cfg.rho = float(rho)
         self.capacity, self.L, self.nf, self.nmodes = int(B), int(L), int(nf), int(nmodes)
         self.single_precision = bool(single_precision)
         self.scan = scan

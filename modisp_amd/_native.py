@@ -23,7 +23,7 @@ class Config(ctypes.Structure):
                 ("single_precision", ctypes.c_int), ("group_velocity", ctypes.c_int),
                 ("group_rtol", ctypes.c_double), ("continuation", ctypes.c_int), ("anchor_stride", ctypes.c_int),
                 ("nc", ctypes.c_int), ("tau_floor", ctypes.c_double), ("tau_phase", ctypes.c_double),
-                ("chunk_traces", ctypes.c_int)]
+                ("chunk_traces", ctypes.c_int), ("grid", ctypes.c_int), ("rho", ctypes.c_double)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)

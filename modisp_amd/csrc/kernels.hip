@@ -64,235 +64,352 @@ __device__ __forceinline__ Layer<T> make_layer(const PrepRow &r, double omega, d
   return l;
 }
 
-// Secular F from prepared rows held in global memory (FP64 lane-per-item work).
+// Secular F from prepared rows held in global memory (FP64 lane-per-item
+// work); optionally the summed log2 normalization scale.
 __device__ __forceinline__ double secular_rows(const PrepRow *rows, int L, int top, int wave, double omega,
-                                               double c) {
+                                               double c, float *log_scale = nullptr) {
   const double k = omega / c, iw2 = 1. / (omega * omega);
+  float ls = 0.f;
+  double F;
   if (wave == 0) {
     const double k2 = k * k;
     double e[5];
-    rayleigh_halfspace(make_layer<double>(rows[L - 1], omega, iw2, 0), k, k2, e);
-    for (int j = L - 2; j >= top; --j) rayleigh_layer(make_layer<double>(rows[j], omega, iw2, 0), k, k2, e);
-    if (top) {
-      Layer<double> wl = make_layer<double>(rows[0], omega, iw2, 0);
-      return rayleigh_water_top(wl, k, e);
+    double m = rayleigh_halfspace(make_layer<double>(rows[L - 1], omega, iw2, 0), k, k2, e);
+    if (log_scale) ls += scale_log2(m);
+    for (int j = L - 2; j >= top; --j) {
+      m = rayleigh_layer(make_layer<double>(rows[j], omega, iw2, 0), k, k2, e);
+      if (log_scale) ls += scale_log2(m);
     }
-    return e[0];
+    F = top ? rayleigh_water_top(make_layer<double>(rows[0], omega, iw2, 0), k, e) : e[0];
+  } else {
+    const Layer<double> hs = make_layer<double>(rows[L - 1], omega, iw2, 1);
+    double e[2] = {hs.rho * m_sqrt((k + hs.kb) * m_fabs(k - hs.kb)), 1.};
+    double m = rescale<2>(e);
+    if (log_scale) ls += scale_log2(m);
+    for (int j = L - 2; j >= top; --j) {
+      m = love_layer(make_layer<double>(rows[j], omega, iw2, 1), k, e);
+      if (log_scale) ls += scale_log2(m);
+    }
+    F = e[0];
   }
-  const Layer<double> hs = make_layer<double>(rows[L - 1], omega, iw2, 1);
-  double e[2] = {hs.rho * m_sqrt((k + hs.kb) * m_fabs(k - hs.kb)), 1.};
-  rescale<2>(e);
-  for (int j = L - 2; j >= top; --j) love_layer(make_layer<double>(rows[j], omega, iw2, 1), k, e);
+  if (log_scale) *log_scale = ls;
+  return F;
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers (one wavefront owns one trace; no block barriers).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+// Secular F with layer constants formed on the fly from global rows (models
+// too deep for an LDS layer table).
+template <class T>
+__device__ __forceinline__ T secular_global(const PrepRow *rows, int L, int top, int wave, double omega,
+                                            double iw2, T k, float *log_scale) {
+  float ls = 0.f;
+  if (wave == 0) {
+    const T k2 = k * k;
+    T e[5];
+    ls += scale_log2(rayleigh_halfspace(make_layer<T>(rows[L - 1], omega, iw2, 0), k, k2, e));
+    for (int j = L - 2; j >= top; --j)
+      ls += scale_log2(rayleigh_layer(make_layer<T>(rows[j], omega, iw2, 0), k, k2, e));
+    *log_scale = ls;
+    return top ? rayleigh_water_top(make_layer<T>(rows[0], omega, iw2, 0), k, e) : e[0];
+  }
+  const Layer<T> hs = make_layer<T>(rows[L - 1], omega, iw2, 1);
+  T e[2] = {hs.rho * m_sqrt((k + hs.kb) * m_fabs(k - hs.kb)), T(1)};
+  ls += scale_log2(rescale<2>(e));
+  for (int j = L - 2; j >= top; --j) ls += scale_log2(love_layer(make_layer<T>(rows[j], omega, iw2, 1), k, e));
+  *log_scale = ls;
   return e[0];
 }
 
 // ---------------------------------------------------------------------------
-// Block-wide inclusive scan of four counters (256 lanes = 4 wave64s).
+// Scan: one wavefront per trace.
+//
+// Nodes. Uniform grid: nc nodes lo + j (hi - lo)/(nc - 1). Adaptive grid:
+// a table of kFloor uniform points plus every layer speed (vs, and vp for
+// Rayleigh) inside the window, each table cell subdivided so that it holds
+// at most 1/rho of the WKB mode count N(c) = (1/pi) sum_j h_j (sqrt(kb_j^2 -
+// k^2)_+ + sqrt(ka_j^2 - k^2)_+) (the reference's density_base / T,
+// cpp/common/physics.hpp:130-138); a cell that starts at a layer speed gets
+// quadratically spaced nodes, matching N ~ sqrt(c - v) there.
+//
+// Nodes are evaluated 64 at a time (62 new per batch, 2 overlap so every
+// classified lane sees both neighbours). Ballots classify certain sign
+// changes, FP32-uncertain nodes and same-sign troughs; popcount prefixes
+// compact them into the trace's ordered event list. The scan stops after the
+// K-th certain sign change, when no later node can change the first K roots.
 // ---------------------------------------------------------------------------
-struct Count4 {
-  int ev, br, un, tr;
-};
-__device__ __forceinline__ Count4 operator+(Count4 a, Count4 b) {
-  return {a.ev + b.ev, a.br + b.br, a.un + b.un, a.tr + b.tr};
-}
-__device__ __forceinline__ Count4 shfl_up4(Count4 v, int d) {
-  return {__shfl_up(v.ev, d, 64), __shfl_up(v.br, d, 64), __shfl_up(v.un, d, 64), __shfl_up(v.tr, d, 64)};
-}
-// Returns the inclusive prefix; total is written to *total (all lanes).
-__device__ __forceinline__ Count4 block_scan4(Count4 v, Count4 *wave_tot, Count4 *total) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const Count4 u = shfl_up4(v, d);
-    if (lane >= d) v = v + u;
-  }
-  if (lane == 63) wave_tot[wave] = v;
-  __syncthreads();
-  Count4 off{0, 0, 0, 0}, tot{0, 0, 0, 0};
-#pragma unroll
-  for (int w = 0; w < kBlock / 64; ++w) {
-    const Count4 t = wave_tot[w];
-    if (w < wave) off = off + t;
-    tot = tot + t;
-  }
-  __syncthreads();
-  *total = tot;
-  return v + off;
-}
+constexpr int kWaves = kBlock / 64;
+constexpr int kTabMax = kFloor + 2 * kAdaptRows;
 
-// ---------------------------------------------------------------------------
-// Scan: FP32 (or FP64) uniform grid, classification and event compaction.
-// ---------------------------------------------------------------------------
-template <class T, int S>
-struct ScanShared {
-  Layer<T> lay[kMaxRows];
-  float fval[kBlock * S];
-  unsigned char unc[kBlock * S];
-  Count4 wave_tot[kBlock / 64];
-  int cut_node, q_unc, q_tr;
+template <class T>
+struct WaveScratch {
+  Layer<T> lay[kAdaptRows];
+  double tc[kTabMax];        // table coordinates, ascending
+  double ks[2 * kAdaptRows];  // layer speeds inside the window (unsorted)
+  float tn[kTabMax];         // N at table points
+  int pre[kTabMax];          // node index of table point m
+  unsigned char kink[kTabMax];
 };
 
-// Scans trace t with the whole block. Every return is block-uniform.
-template <class T, int S>
-__device__ void scan_trace(const PipeArgs &a, int t, ScanShared<T, S> &sh) {
-  constexpr int NC = kBlock * S;
-  Layer<T> *lay = sh.lay;
-  float *fval = sh.fval;
-  unsigned char *unc = sh.unc;
-  Count4 *wave_tot = sh.wave_tot;
+template <class T>
+__device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane) {
   const TraceInfo ti = a.traces[t];
   const ModelInfo mi = a.models[ti.model];
-  const int L = mi.L, top = mi.top, K = mi.K;
-  const int tid = threadIdx.x;
-  const double omega = ti.omega, iw2 = 1. / (omega * omega);
-  const PrepRow *rows = a.rows + mi.row0;
-  for (int j = tid; j < L; j += kBlock) lay[j] = make_layer<T>(rows[j], omega, iw2, a.wave);
-  Event *ev = a.events + static_cast<size_t>(t - a.trace_begin) * kEventCap;
-  TraceScan *ts = a.scan + (t - a.trace_begin);
+  const int tl = t - a.trace_begin;
+  Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
+  TraceScan *ts = a.scan + tl;
+  constexpr bool fp32 = sizeof(T) == 4;
   if (mi.empty) {
-    if (tid == 0) {
-      ts->nev = 0;
-      ts->flags = 0;
-      ts->status = mi.empty;
-    }
+    if (lane == 0) *ts = TraceScan{0, 0u, mi.empty, 0, 0, 0};
     return;
   }
-  __syncthreads();
-
-  const double lo = mi.lo, dc = (mi.hi - mi.lo) / double(NC - 1);
-  const bool fp32 = sizeof(T) == 4;
-  const float tau = fp32 ? ti.tau : 0.f;
-  const double hi_guard = mi.cutoff_guard;
+  const double lo = mi.lo, hi = mi.hi;
   // Far below the largest shear speed the FP32 compound matrix cancels
   // (reference limit: max Vs / 16); such windows go straight to the FP64 grid.
   if (fp32 && lo < mi.stable_low) {
-    if (tid == 0) {
-      ts->nev = 0;
-      ts->flags = kTraceOverflow;
-      ts->status = 0;
-      a.overflow_list[atomicAdd(a.overflow_count, 1)] = t - a.trace_begin;
+    if (lane == 0) {
+      *ts = TraceScan{0, kTraceOverflow, 0, 0, 0, 0};
+      a.overflow_list[atomicAdd(a.overflow_count, 1)] = tl;
     }
     return;
   }
-  // Evaluate: sample i = s*kBlock + tid (consecutive lanes, consecutive samples).
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int i = s * kBlock + tid;
-    const double c = lo + dc * double(i);
-    const T k = T(omega / c);
-    const T f = a.wave == 0 ? rayleigh_F<T>(lay, L, top, k) : love_F<T>(lay, L, top, k);
-    const float fv = float(f);
-    fval[i] = fv;
-    unc[i] = (fp32 && (!(fabsf(fv) >= tau) || c >= hi_guard)) ? 1 : 0;
-  }
-  __syncthreads();
-
-  // Classify the contiguous nodes [tid*S, tid*S + S) of this lane.
-  const float theta = kTroughMax;
-  unsigned char kind[S];
-  Count4 cnt{0, 0, 0, 0};
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int i = tid * S + s;
-    const float fi = fval[i];
-    const bool ui = unc[i];
-    unsigned char k8 = kNone;
-    if (ui) k8 = kUncertain;
-    else if (i + 1 < NC && !unc[i + 1] && (signbit(fi) != signbit(fval[i + 1]))) k8 = kBracket;
-    else if (i >= 1 && i + 1 < NC && !unc[i - 1] && !unc[i + 1]) {
-      const float fl = fval[i - 1], fr = fval[i + 1];
-      const float ai = fabsf(fi);
-      if (signbit(fl) == signbit(fi) && signbit(fr) == signbit(fi) && ai < fabsf(fl) && ai < fabsf(fr) &&
-          ai < theta)
-        k8 = kTrough;
+  const int L = mi.L, top = mi.top, K = mi.K;
+  const double omega = ti.omega, iw2 = 1. / (omega * omega);
+  const PrepRow *rows = a.rows + mi.row0;
+  const bool in_lds = L <= kAdaptRows;
+  if (in_lds)
+    for (int j = lane; j < L; j += 64) sh.lay[j] = make_layer<T>(rows[j], omega, iw2, a.wave);
+  wave_sync();
+  const bool adaptive = a.adaptive && in_lds;
+  int total = a.nc, ntab = 0;
+  const double dc = (hi - lo) / double(a.nc - 1);
+  if (adaptive) {
+    // Layer speeds strictly inside the window (row `lane`; L <= 64).
+    double v0 = 0., v1 = 0.;
+    int nmine = 0;
+    if (lane < L - 1 && rows[lane].h > 0.) {
+      const PrepRow r = rows[lane];
+      if (r.ivs > 0.) {
+        const double v = 1. / r.ivs;
+        if (v > lo && v < hi) { v0 = v; ++nmine; }
+      }
+      if (a.wave == 0) {
+        const double v = 1. / r.ivp;
+        if (v > lo && v < hi) {
+          if (nmine) v1 = v; else v0 = v;
+          ++nmine;
+        }
+      }
     }
-    kind[s] = k8;
-    cnt.ev += k8 != kNone;
-    cnt.br += k8 == kBracket;
-    cnt.un += k8 == kUncertain;
-    cnt.tr += k8 == kTrough;
+    const int incl = wave_incl_scan(nmine, lane);
+    const int nk = __shfl(incl, 63, 64);
+    const int off = incl - nmine;
+    if (nmine > 0) sh.ks[off] = v0;
+    if (nmine > 1) sh.ks[off + 1] = v1;
+    wave_sync();
+    const double dt = (hi - lo) / double(kFloor - 1);
+    auto floor_point = [&](int m) { return m == kFloor - 1 ? hi : lo + dt * double(m); };
+    auto density = [&](double c) {
+      const float k = float(omega / c), k2 = k * k;
+      float n = 0.f;
+      for (int j = top; j < L - 1; ++j) {
+        const float h = float(sh.lay[j].h);
+        const float kb = float(sh.lay[j].kb), ka = float(sh.lay[j].ka);
+        n += h * sqrtf(fmaxf(0.f, kb * kb - k2));
+        if (a.wave == 0) n += h * sqrtf(fmaxf(0.f, ka * ka - k2));
+      }
+      if (top && a.wave == 0) {  // acoustic water layer: P term only
+        const float h = float(sh.lay[0].h), ka = float(sh.lay[0].ka);
+        n += h * sqrtf(fmaxf(0.f, ka * ka - k2));
+      }
+      return n * 0.3183098861837907f;  // 1/pi
+    };
+    // Merge the kFloor uniform points with the layer speeds by rank.
+    {
+      const double tm = floor_point(lane);
+      int below = 0;
+      for (int r = 0; r < nk; ++r) below += sh.ks[r] < tm;
+      sh.tc[lane + below] = tm;
+      sh.kink[lane + below] = 0;
+      sh.tn[lane + below] = density(tm);
+    }
+    for (int q = 0; q < nmine; ++q) {
+      const double v = q ? v1 : v0;
+      const int me = off + q;
+      int rank = 0;
+      for (int r = 0; r < nk; ++r) {
+        const double u = sh.ks[r];
+        rank += (u < v) || (u == v && r < me);
+      }
+      int ubelow = 0;
+      for (int m = 0; m < kFloor; ++m) ubelow += floor_point(m) <= v;
+      sh.tc[rank + ubelow] = v;
+      sh.kink[rank + ubelow] = 1;
+      sh.tn[rank + ubelow] = density(v);
+    }
+    wave_sync();
+    // Density nodes per table cell; node index of every table point.
+    ntab = kFloor + nk;
+    int carry = 0;
+    for (int base = 0; base < ntab; base += 64) {
+      const int m = base + lane;
+      int n = 0;
+      if (m + 1 < ntab && sh.tc[m + 1] > sh.tc[m]) {
+        const float want = ceilf(a.rho * (sh.tn[m + 1] - sh.tn[m])) - 1.f;
+        n = want > 0.f ? (want < float(kCellMax) ? int(want) : kCellMax) : 0;
+      }
+      const int inc = wave_incl_scan(n, lane);
+      if (m < ntab) sh.pre[m] = m + carry + inc - n;
+      carry += __shfl(inc, 63, 64);
+    }
+    wave_sync();
+    total = sh.pre[ntab - 1] + 1;
   }
-  Count4 total;
-  const Count4 incl = block_scan4(cnt, wave_tot, &total);
-  const Count4 excl{incl.ev - cnt.ev, incl.br - cnt.br, incl.un - cnt.un, incl.tr - cnt.tr};
-  // Cut after the K-th certain bracket (the first K roots lie at or before it).
-  if (tid == 0) sh.cut_node = NC;
-  __syncthreads();
-  if (excl.br < K && incl.br >= K) {
-    int b = excl.br;
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-      if (kind[s] == kBracket && ++b == K) sh.cut_node = tid * S + s;
+  // Coordinate of node j (ascending in j).
+  auto node_c = [&](int j) -> double {
+    if (!adaptive) return lo + dc * double(j);
+    int l = 0, h = ntab - 1;  // largest m with pre[m] <= j
+    while (l < h) {
+      const int mid = (l + h + 1) >> 1;
+      if (sh.pre[mid] <= j) l = mid; else h = mid - 1;
+    }
+    const int i = j - sh.pre[l];
+    if (i == 0) return sh.tc[l];
+    const int n = sh.pre[l + 1] - sh.pre[l];  // interior nodes + 1
+    const double s = double(i) / double(n);
+    return sh.tc[l] + (sh.tc[l + 1] - sh.tc[l]) * (sh.kink[l] ? s * s : s);
+  };
+  const float tau = fp32 ? ti.tau : 0.f;
+  const double hi_guard = mi.cutoff_guard;
+  const float theta = kTroughMax;
+  int nb = 0, ne = 0;
+  bool overflow = false;
+  for (int base = 0;; base += 62) {
+    const int j = base + lane;
+    const bool active = j < total;
+    double c = 0.;
+    float f = 0.f, sc = 0.f;
+    bool u = false;
+    if (active) {
+      c = node_c(j);
+      const T k = T(omega / c);
+      T F;
+      if (in_lds)
+        F = a.wave == 0 ? rayleigh_F_scale<T>(sh.lay, L, top, k, &sc) : love_F_scale<T>(sh.lay, L, top, k, &sc);
+      else
+        F = secular_global<T>(rows, L, top, a.wave, omega, iw2, k, &sc);
+      f = float(F);
+      u = fp32 && (!(fabsf(f) >= tau) || c >= hi_guard);
+    }
+    const float fl = __shfl(f, (lane + 63) & 63, 64), fr = __shfl(f, (lane + 1) & 63, 64);
+    const float sl = __shfl(sc, (lane + 63) & 63, 64), sr = __shfl(sc, (lane + 1) & 63, 64);
+    const double cl = __shfl(c, (lane + 63) & 63, 64), cr = __shfl(c, (lane + 1) & 63, 64);
+    const uint64_t um = __ballot(u);
+    const bool ul = (um >> ((lane + 63) & 63)) & 1, ur = (um >> ((lane + 1) & 63)) & 1;
+    const bool has_left = j >= 1, has_right = j + 1 < total;
+    // Lanes 1..62 see both neighbours; lane 0 only at the first node.
+    const bool classify = active && (lane >= 1 || base == 0) && lane <= 62;
+    int kind = kNone;
+    if (classify) {
+      if (u) kind = kUncertain;
+      else if (has_right && !ur && signbit(f) != signbit(fr)) kind = kBracket;
+      else if (has_left && has_right && !ul && !ur && signbit(fl) == signbit(f) && signbit(fr) == signbit(f)) {
+        // Same-sign window: a hidden root pair shows as a small local minimum
+        // of |F| (trough) or as a dip of the log2 normalization scale (a
+        // collapsing state vector between two very close roots).
+        const float af = fabsf(f);
+        if (af < fabsf(fl) && af < fabsf(fr) && af < theta) kind = kTrough;
+        else if (sc < sl && sc < sr && 0.5f * (sl + sr) - sc > kDipDepth) kind = kDip;
+      }
+    }
+    const uint64_t bm = __ballot(kind == kBracket);
+    uint64_t em = __ballot(kind != kNone);
+    bool done = total - 1 - base <= 62;
+    const int nbb = __popcll(bm);
+    if (nb + nbb >= K) {  // cut at the K-th certain sign change
+      uint64_t m = bm;
+      for (int r = nb + 1; r < K; ++r) m &= m - 1;
+      const int cut = __builtin_ctzll(m);
+      em &= cut == 63 ? ~0ull : ((2ull << cut) - 1ull);
+      done = true;
+    }
+    nb += nbb;
+    const int nnew = __popcll(em);
+    if (ne + nnew > kEventCap) {
+      overflow = true;
+      break;
+    }
+    const bool mine = (em >> lane) & 1;
+    const uint64_t uqm = em & __ballot(kind == kUncertain), tqm = em & __ballot(kind == kTrough || kind == kDip);
+    int qu = 0, qt = 0;
+    if (lane == 0) {
+      if (uqm) qu = atomicAdd(a.unc_count, __popcll(uqm));
+      if (tqm) qt = atomicAdd(a.trough_count, __popcll(tqm));
+    }
+    qu = __shfl(qu, 0, 64);
+    qt = __shfl(qt, 0, 64);
+    if (mine) {
+      const int slot = ne + __popcll(em & lanes_below(lane));
+      Event x;
+      x.c0 = has_left ? cl : NAN;
+      x.c1 = c;
+      x.c2 = has_right ? cr : NAN;
+      x.v = 0.;
+      x.x = NAN;
+      x.fl = fl;
+      x.fc = f;
+      x.fr = fr;
+      x.node = j;
+      x.kind = kind;
+      x.sc = sc;
+      ev[slot] = x;
+      const uint32_t g = static_cast<uint32_t>(tl * kEventCap + slot);
+      if (kind == kUncertain) a.unc_queue[qu + __popcll(uqm & lanes_below(lane))] = g;
+      if (kind == kTrough || kind == kDip) a.trough_queue[qt + __popcll(tqm & lanes_below(lane))] = g;
+    }
+    ne += nnew;
+    if (done) break;
   }
-  __syncthreads();
-  const int cut = sh.cut_node;
-  // Events at or before the cut, and how many of them are uncertain / troughs.
-  Count4 kept{0, 0, 0, 0};
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int i = tid * S + s;
-    const bool keep = kind[s] != kNone && i <= cut;
-    kept.ev += keep;
-    kept.un += keep && kind[s] == kUncertain;
-    kept.tr += keep && kind[s] == kTrough;
-  }
-  Count4 ktot;
-  const Count4 kin = block_scan4(kept, wave_tot, &ktot);
-  const bool overflow = ktot.ev > kEventCap;
-  if (tid == 0) {
-    sh.q_unc = (!overflow && ktot.un) ? atomicAdd(a.unc_count, ktot.un) : 0;
-    sh.q_tr = (!overflow && ktot.tr) ? atomicAdd(a.trough_count, ktot.tr) : 0;
-    ts->nev = overflow ? 0 : ktot.ev;
-    ts->flags = overflow ? kTraceOverflow : 0u;
-    ts->status = 0;
-    ts->certain = total.br;
+  if (lane == 0) {
+    *ts = TraceScan{overflow ? 0 : ne, overflow ? kTraceOverflow : 0u, 0, nb, 0, total};
     // An FP32 trace whose events overflow is rescanned on the FP64 grid,
     // which has no uncertain nodes.
-    if (overflow && fp32) a.overflow_list[atomicAdd(a.overflow_count, 1)] = t - a.trace_begin;
-  }
-  __syncthreads();
-  if (overflow) return;
-  const int q_unc = sh.q_unc, q_tr = sh.q_tr;
-  int e = kin.ev - kept.ev, u = kin.un - kept.un, r = kin.tr - kept.tr;
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int i = tid * S + s;
-    if (kind[s] == kNone || i > cut) continue;
-    Event x;
-    x.node = i;
-    x.kind = kind[s];
-    x.fl = i > 0 ? fval[i - 1] : 0.f;
-    x.fc = fval[i];
-    x.fr = i + 1 < NC ? fval[i + 1] : 0.f;
-    x.v = 0.;
-    x.x = 0.;
-    if (kind[s] == kBracket) x.fr = fval[i + 1];
-    ev[e] = x;
-    const uint32_t gidx = static_cast<uint32_t>((t - a.trace_begin) * kEventCap + e);
-    if (kind[s] == kUncertain && q_unc + u < a.queue_cap) a.unc_queue[q_unc + u] = gidx;
-    if (kind[s] == kTrough && q_tr + r < a.queue_cap) a.trough_queue[q_tr + r] = gidx;
-    u += kind[s] == kUncertain;
-    r += kind[s] == kTrough;
-    ++e;
+    if (overflow && fp32) a.overflow_list[atomicAdd(a.overflow_count, 1)] = tl;
   }
 }
 
-template <class T, int S>
+template <class T>
 __global__ __launch_bounds__(kBlock) void scan_kernel(PipeArgs a) {
-  __shared__ ScanShared<T, S> sh;
-  const int t = blockIdx.x + a.trace_begin;
-  if (t < a.trace_end) scan_trace<T, S>(a, t, sh);
+  __shared__ WaveScratch<T> scratch[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int t = a.trace_begin + blockIdx.x * kWaves + w;
+  if (t < a.trace_end) scan_wave<T>(a, t, scratch[w], lane);
 }
 
-// FP64 rescan of the traces whose FP32 event list overflowed (persistent).
-template <int S>
+// FP64 rescan of the traces whose FP32 event list overflowed (persistent,
+// one wavefront per listed trace).
 __global__ __launch_bounds__(kBlock) void rescan_kernel(PipeArgs a) {
-  __shared__ ScanShared<double, S> sh;
+  __shared__ WaveScratch<double> scratch[kWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int n = *a.overflow_count;
-  for (int q = blockIdx.x; q < n; q += gridDim.x) {
-    scan_trace<double, S>(a, a.overflow_list[q] + a.trace_begin, sh);
-    __syncthreads();
-  }
+  for (int q = blockIdx.x * kWaves + w; q < n; q += gridDim.x * kWaves)
+    scan_wave<double>(a, a.overflow_list[q] + a.trace_begin, scratch[w], lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -306,9 +423,7 @@ __global__ __launch_bounds__(kBlock) void recheck_kernel(PipeArgs a) {
     Event &x = a.events[g];
     const TraceInfo ti = a.traces[tl + a.trace_begin];
     const ModelInfo mi = a.models[ti.model];
-    const double dc = (mi.hi - mi.lo) / double(a.nc - 1);
-    const double c = mi.lo + dc * double(x.node);
-    x.v = secular_rows(a.rows + mi.row0, mi.L, mi.top, a.wave, ti.omega, c);
+    x.v = secular_rows(a.rows + mi.row0, mi.L, mi.top, a.wave, ti.omega, x.c1);
   }
 }
 
@@ -322,28 +437,42 @@ __global__ __launch_bounds__(kBlock) void trough_kernel(PipeArgs a) {
     const uint32_t g = a.trough_queue[q];
     const int tl = g / kEventCap;
     Event &x = a.events[g];
+    if (x.kind != kTrough && x.kind != kDip) continue;
+    const bool dip = x.kind == kDip;
     const TraceInfo ti = a.traces[tl + a.trace_begin];
     const ModelInfo mi = a.models[ti.model];
     const PrepRow *rows = a.rows + mi.row0;
-    const double dc = (mi.hi - mi.lo) / double(a.nc - 1);
-    double lo = mi.lo + dc * double(x.node - 1), hi = mi.lo + dc * double(x.node + 1);
+    double lo = x.c0, hi = x.c2;
     const double sign = x.fc < 0.f ? -1. : 1.;
+    // Golden-section minimization of sign*F (trough) or of the log2 scale
+    // (dip), stopping at the first point where F takes the opposite sign.
+    auto probe = [&](double c, float &key) {
+      float ls;
+      const double f = sign * secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, c, &ls);
+      key = dip ? ls : float(f);
+      return f;
+    };
     constexpr double phi = 0.6180339887498949;
     double p = hi - phi * (hi - lo), r = lo + phi * (hi - lo);
-    double fp = sign * secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, p);
-    double fr = sign * secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, r);
+    float kp, kr;
+    double fp = probe(p, kp), fr = probe(r, kr);
+    float kmin = fminf(kp, kr);
     double split = NAN, fsplit = 0.;
     for (int it = 0; it < kTroughIters; ++it) {
       if (fp <= 0.) { split = p; fsplit = sign * fp; break; }
       if (fr <= 0.) { split = r; fsplit = sign * fr; break; }
-      if (fp < fr) {
-        hi = r; r = p; fr = fp;
+      // A dip that does not deepen like log|c - c*| is not a collapsing resonance.
+      if (dip && it >= kDipStopIter && kmin > x.sc - kDipStopDrop) break;
+      if (kp < kr) {
+        hi = r; r = p; fr = fp; kr = kp;
         p = hi - phi * (hi - lo);
-        fp = sign * secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, p);
+        fp = probe(p, kp);
+        kmin = fminf(kmin, kp);
       } else {
-        lo = p; p = r; fp = fr;
+        lo = p; p = r; fp = fr; kp = kr;
         r = lo + phi * (hi - lo);
-        fr = sign * secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, r);
+        fr = probe(r, kr);
+        kmin = fminf(kmin, kr);
       }
     }
     x.x = split;
@@ -364,49 +493,43 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(PipeArgs a) {
   Bracket *out = a.brackets + static_cast<size_t>(tl) * a.kmax;
   const int K = mi.K;
   int nb = 0;
-  unsigned flags = ts.flags;
+  const unsigned flags = ts.flags;
   if (ts.status == 0 && !(flags & kTraceOverflow)) {
     const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
-    const double dc = (mi.hi - mi.lo) / double(a.nc - 1);
-    auto node_c = [&](int i) { return mi.lo + dc * double(i); };
     auto emit = [&](double lo, double hi, double flo, double fhi) {
       if (nb < a.kmax) out[nb] = Bracket{lo, hi, flo, fhi};
       ++nb;
     };
     int prev_unc = -2;
     double prev_v = 0.;
-    const bool fp32 = a.fp32;
     for (int e = 0; e < ts.nev && nb < K; ++e) {
       const Event x = ev[e];
-      const int i = x.node;
       if (x.kind == kBracket) {
-        emit(node_c(i), node_c(i + 1), x.fc, x.fr);
-      } else if (x.kind == kTrough) {
-        if (x.x == x.x) {  // split found
-          emit(node_c(i - 1), x.x, x.fl, x.v);
-          if (nb < K) emit(x.x, node_c(i + 1), x.v, x.fr);
+        emit(x.c1, x.c2, x.fc, x.fr);
+      } else if (x.kind == kTrough || x.kind == kDip) {
+        if (x.x == x.x) {  // hidden pair split at x
+          emit(x.c0, x.x, x.fl, x.v);
+          if (nb < K) emit(x.x, x.c2, x.v, x.fr);
         }
-      } else {  // uncertain node with FP64 value x.v
+      } else {  // uncertain node with its FP64 value; owns its left cell when
+                // that neighbour is certain and always its right cell
         const double v = x.v;
-        if (i > 0) {
-          const bool left_unc = prev_unc == i - 1;
-          const double fl = left_unc ? prev_v : double(x.fl);
-          if (signbit(fl) != signbit(v) && nb < K) emit(node_c(i - 1), node_c(i), fl, v);
+        if (x.c0 == x.c0) {
+          const double fl = prev_unc == x.node - 1 ? prev_v : double(x.fl);
+          if (signbit(fl) != signbit(v) && nb < K) emit(x.c0, x.c1, fl, v);
         }
-        const bool right_unc = e + 1 < ts.nev && ev[e + 1].kind == kUncertain && ev[e + 1].node == i + 1;
-        if (!right_unc && i + 1 < a.nc && nb < K) {
+        const bool right_unc = e + 1 < ts.nev && ev[e + 1].kind == kUncertain && ev[e + 1].node == x.node + 1;
+        if (!right_unc && x.c2 == x.c2 && nb < K) {
           const double fr = double(x.fr);
-          if (signbit(v) != signbit(fr)) emit(node_c(i), node_c(i + 1), v, fr);
+          if (signbit(v) != signbit(fr)) emit(x.c1, x.c2, v, fr);
         }
-        prev_unc = i;
+        prev_unc = x.node;
         prev_v = v;
       }
     }
-    (void)fp32;
   }
   const int count = nb < K ? nb : K;
   ts.count = (flags & kTraceOverflow) ? -1 : (ts.status ? 0 : count);
-  ts.flags = flags;
 }
 
 // ---------------------------------------------------------------------------
@@ -581,6 +704,17 @@ __global__ void reset_kernel(int *counters, int n) {
   if (threadIdx.x < n) counters[threadIdx.x] = 0;
 }
 
+// Accumulate this chunk's work counters into the run statistics:
+// [0] traces, [1] FP64 rechecks, [2] trough/dip probes, [3] FP64 rescans.
+__global__ void stats_kernel(const int *counters, unsigned long long *stats, int traces) {
+  if (threadIdx.x == 0) {
+    stats[0] += static_cast<unsigned long long>(traces);
+    stats[1] += static_cast<unsigned long long>(counters[0]);
+    stats[2] += static_cast<unsigned long long>(counters[1]);
+    stats[3] += static_cast<unsigned long long>(counters[2]);
+  }
+}
+
 }  // namespace mdr
 
 // ---------------------------------------------------------------------------
@@ -589,25 +723,10 @@ __global__ void reset_kernel(int *counters, int n) {
 namespace mdr {
 
 template <class T>
-static hipError_t launch_scan_t(const PipeArgs &a, int nc, hipStream_t s) {
+static hipError_t launch_scan_t(const PipeArgs &a, hipStream_t s) {
   const int n = a.trace_end - a.trace_begin;
   if (n <= 0) return hipSuccess;
-  switch (nc) {
-    case 256: hipLaunchKernelGGL((scan_kernel<T, 1>), dim3(n), dim3(kBlock), 0, s, a); break;
-    case 512: hipLaunchKernelGGL((scan_kernel<T, 2>), dim3(n), dim3(kBlock), 0, s, a); break;
-    case 1024: hipLaunchKernelGGL((scan_kernel<T, 4>), dim3(n), dim3(kBlock), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-static hipError_t launch_rescan(const PipeArgs &a, int nc, int blocks, hipStream_t s) {
-  switch (nc) {
-    case 256: hipLaunchKernelGGL((rescan_kernel<1>), dim3(blocks), dim3(kBlock), 0, s, a); break;
-    case 512: hipLaunchKernelGGL((rescan_kernel<2>), dim3(blocks), dim3(kBlock), 0, s, a); break;
-    case 1024: hipLaunchKernelGGL((rescan_kernel<4>), dim3(blocks), dim3(kBlock), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-  }
+  hipLaunchKernelGGL((scan_kernel<T>), dim3((n + kWaves - 1) / kWaves), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -620,11 +739,11 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
   };
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, sa.counters, 4);
   mark(0);
-  hipError_t err = fp32 ? launch_scan_t<float>(sa, pa.nc, s) : launch_scan_t<double>(sa, pa.nc, s);
+  hipError_t err = fp32 ? launch_scan_t<float>(sa, s) : launch_scan_t<double>(sa, s);
   if (err != hipSuccess) return err;
   mark(1);
   const int pb = persistent_blocks;
-  if (fp32 && (err = launch_rescan(pa, pa.nc, pb, s)) != hipSuccess) return err;
+  if (fp32) hipLaunchKernelGGL(rescan_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   mark(2);
   hipLaunchKernelGGL(recheck_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   hipLaunchKernelGGL(trough_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
@@ -636,6 +755,7 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
   mark(5);
   if (group) hipLaunchKernelGGL(group_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   mark(6);
+  if (pa.stats) hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(64), 0, s, pa.counters, pa.stats, n);
   return hipGetLastError();
 }
 

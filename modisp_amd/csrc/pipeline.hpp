@@ -19,10 +19,16 @@ namespace mdr {
 constexpr int kBlock = 256;       // lanes per scan block (4 wave64)
 constexpr int kMaxRows = 256;     // layer rows per model (reference limit, __init__.py:152)
 constexpr int kEventCap = 32;     // ascending events kept per trace
+constexpr int kFloor = 64;        // uniform table points of the adaptive grid (one per lane)
+constexpr int kAdaptRows = 64;    // rows held in an LDS layer table / eligible for the adaptive grid
+constexpr int kCellMax = 1 << 16; // density nodes per table cell (cap)
 constexpr float kTroughMax = 0.05f;  // |F| below which a same-sign trough is probed
-constexpr int kTroughIters = 48;  // golden-section steps per trough probe
+constexpr int kTroughIters = 28;  // golden-section steps per trough / dip probe
+constexpr float kDipDepth = 0.5f; // log2-scale dip (bits below the neighbours' mean) that is probed
+constexpr int kDipStopIter = 6;   // a dip probe gives up after this many steps ...
+constexpr float kDipStopDrop = 1.5f;  // ... unless the log2 scale fell this far below the node's
 
-enum EventKind : int { kNone = 0, kBracket = 1, kUncertain = 2, kTrough = 3 };
+enum EventKind : int { kNone = 0, kBracket = 1, kUncertain = 2, kTrough = 3, kDip = 4 };
 
 // Trace-flag bits (reference python/modisp/__init__.py:15-32).
 constexpr unsigned kFlagUnconverged = 4u, kFlagNearCutoff = 16u, kFlagRefineFailed = 64u,
@@ -72,21 +78,22 @@ struct TraceInfo {
 };
 
 struct Event {
-  int node;   // grid node (cell node..node+1 for brackets)
+  double c0, c1, c2;  // nodes j-1, j, j+1 (NaN where absent); a bracket is [c1, c2]
+  double v;           // FP64 value at c1 (uncertain) / F at the split (trough)
+  double x;           // trough split point (NaN: none)
+  float fl, fc, fr;   // scan values at c0, c1, c2
+  int node;           // node index j within the trace
   int kind;
-  float fl, fc, fr;  // FP32 values at node-1, node, node+1
-  float pad;
-  double v;   // FP64 value (uncertain) / F at split (trough)
-  double x;   // trough split point (NaN: none)
+  float sc;           // summed log2 normalization scale at c1
 };
 
 struct TraceScan {
   int nev;
   unsigned flags;
   int status;   // ModelInfo::empty for empty traces
-  int certain;  // certain brackets seen on the whole grid
+  int certain;  // certain sign changes seen by the scan
   int count;    // roots found (<= K), -1 unresolved
-  int pad;
+  int nodes;    // grid nodes of the trace
 };
 
 struct Bracket {
@@ -111,6 +118,8 @@ struct PipeArgs {
   int queue_cap;
   int trace_begin, trace_end;
   int nc;
+  int adaptive;   // 1: density-adaptive grid, 0: uniform nc-node grid
+  float rho;      // adaptive grid: nodes per WKB mode
   int kmax;
   int wave;
   int fp32;
@@ -123,6 +132,7 @@ struct PipeArgs {
   int32_t *counts;
   double *group;
   uint32_t *group_flags;
+  unsigned long long *stats;  // optional run statistics (see stats_kernel)
 };
 using ScanArgs = PipeArgs;
 

@@ -107,7 +107,7 @@ template <> MDR_HD double tiny_scale<double>() { return 1.0e-40; }
 // is positive, so neither the sign nor the zeros of F change; it is continuous
 // in the inputs, so float and double evaluations stay directly comparable.
 template <int N, class T>
-MDR_HD void rescale(T *e) {
+MDR_HD T rescale(T *e) {
   T m = T(0);
 #pragma unroll
   for (int i = 0; i < N; ++i) m = fmax(m, m_fabs(e[i]));
@@ -115,7 +115,26 @@ MDR_HD void rescale(T *e) {
     const T inv = fast_rcp(m);
 #pragma unroll
     for (int i = 0; i < N; ++i) e[i] *= inv;
+    return m;
   }
+  return T(1);
+}
+
+// log2 of a (positive) normalization scale, as float. A trapped-mode
+// resonance collapses the whole state vector between two close roots; the
+// running sum of these logs then has a log|c - c*| cusp that node samples see
+// even when the root pair is far narrower than the node spacing.
+MDR_HD float scale_log2(float m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __log2f(m);
+#else
+  return std::log2(m);
+#endif
+}
+MDR_HD float scale_log2(double m) {
+  int ex;
+  const double mant = frexp(m, &ex);  // m = mant 2^ex, mant in [0.5, 1)
+  return float(ex) + scale_log2(float(mant));
 }
 
 // P (or S) factor of one layer: cosine-like c, sine/r-like w, r*sine-like x,
@@ -178,7 +197,7 @@ MDR_HD Wave<T> wave_factor(T k, T kx, T h) {
 
 // One Rayleigh layer: e <- e * C(layer), then exact rescale.
 template <class T>
-MDR_HD void rayleigh_layer(const Layer<T> &ly, T k, T k2, T e[5]) {
+MDR_HD T rayleigh_layer(const Layer<T> &ly, T k, T k2, T e[5]) {
   const Wave<T> P = wave_factor(k, ly.ka, ly.h);
   const Wave<T> S = wave_factor(k, ly.kb, ly.h);
   const T a0 = P.att * S.att;
@@ -212,12 +231,12 @@ MDR_HD void rayleigh_layer(const Layer<T> &ly, T k, T k2, T e[5]) {
   e[2] = m_fma(I, e0, m_fma(G, e1, m_fma(K, e2, m_fma(H, e3, J * e4))));
   e[3] = m_fma(C, e0, m_fma(M, e1, m_fma(B, e3, m_fma(E, e4, G * e2t))));
   e[4] = m_fma(Lq, e0, m_fma(C, e1, m_fma(D, e3, m_fma(A, e4, I * e2t))));
-  rescale<5>(e);
+  return rescale<5>(e);
 }
 
 // Half-space five-vector (decaying P and S waves below the stack).
 template <class T>
-MDR_HD void rayleigh_halfspace(const Layer<T> &hs, T k, T k2, T e[5]) {
+MDR_HD T rayleigh_halfspace(const Layer<T> &hs, T k, T k2, T e[5]) {
   const T ra = m_sqrt((k + hs.ka) * m_fabs(k - hs.ka));
   const T rb = m_sqrt((k + hs.kb) * m_fabs(k - hs.kb));
   const T gk = hs.gk, gam = gk * k2, gamm1 = gam - T(1), rh = hs.rho, rarb = ra * rb;
@@ -226,7 +245,7 @@ MDR_HD void rayleigh_halfspace(const Layer<T> &hs, T k, T k2, T e[5]) {
   e[2] = rh * (gamm1 - gk * rarb);
   e[3] = rh * rb;
   e[4] = k2 - rarb;
-  rescale<5>(e);
+  return rescale<5>(e);
 }
 
 // Pressure-release water top over the solid stack: F = cos(p) e0 - rho_w w e1.
@@ -248,17 +267,28 @@ MDR_HD T rayleigh_F(const Layer<T> *lay, int L, int top, T k) {
   return top ? rayleigh_water_top(lay[0], k, e) : e[0];
 }
 
+// As rayleigh_F, also returning the summed log2 of the normalization scales.
+template <class T>
+MDR_HD T rayleigh_F_scale(const Layer<T> *lay, int L, int top, T k, float *log_scale) {
+  const T k2 = k * k;
+  T e[5];
+  float ls = scale_log2(rayleigh_halfspace(lay[L - 1], k, k2, e));
+  for (int j = L - 2; j >= top; --j) ls += scale_log2(rayleigh_layer(lay[j], k, k2, e));
+  *log_scale = ls;
+  return top ? rayleigh_water_top(lay[0], k, e) : e[0];
+}
+
 // Love: one SH layer, state (traction, displacement / mu-scaled). The wave
 // factor already carries z = -r sin q (propagating) or r sinh q (evanescent).
 template <class T>
-MDR_HD void love_layer(const Layer<T> &ly, T k, T e[2]) {
+MDR_HD T love_layer(const Layer<T> &ly, T k, T e[2]) {
   const Wave<T> S = wave_factor(k, ly.kb, ly.h);
   const T mu = ly.rho, imu = ly.irho;  // Love rows store mu and 1/mu here
   const T u = m_fma(e[0], S.c, e[1] * mu * S.x);
   const T v = m_fma(e[0] * S.w, imu, e[1] * S.c);
   e[0] = u;
   e[1] = v;
-  rescale<2>(e);
+  return rescale<2>(e);
 }
 
 // Love F: half-space state (mu r, 1) -- the reference's (rho r, 1/vs^2) times
@@ -271,6 +301,18 @@ MDR_HD T love_F(const Layer<T> *lay, int L, int top, T k) {
   e[1] = T(1);
   rescale<2>(e);
   for (int j = L - 2; j >= top; --j) love_layer(lay[j], k, e);
+  return e[0];
+}
+
+template <class T>
+MDR_HD T love_F_scale(const Layer<T> *lay, int L, int top, T k, float *log_scale) {
+  const Layer<T> &hs = lay[L - 1];
+  T e[2];
+  e[0] = hs.rho * m_sqrt((k + hs.kb) * m_fabs(k - hs.kb));
+  e[1] = T(1);
+  float ls = scale_log2(rescale<2>(e));
+  for (int j = L - 2; j >= top; --j) ls += scale_log2(love_layer(lay[j], k, e));
+  *log_scale = ls;
   return e[0];
 }
 
@@ -319,7 +361,7 @@ MDR_HD Dual2 m_rcp(Dual2 x) {
 MDR_HD Dual2 m_fabs(Dual2 x) { return x.v < 0. ? -x : x; }
 MDR_HD Dual2 m_fma(Dual2 x, Dual2 y, Dual2 z) { return x * y + z; }
 template <int N>
-MDR_HD void rescale(Dual2 *e) {
+MDR_HD Dual2 rescale(Dual2 *e) {
   double m = 0.;
 #pragma unroll
   for (int i = 0; i < N; ++i) m = fmax(m, fabs(e[i].v));
@@ -327,6 +369,8 @@ MDR_HD void rescale(Dual2 *e) {
     const double inv = 1. / m;
 #pragma unroll
     for (int i = 0; i < N; ++i) e[i] *= inv;
+    return Dual2(m);
   }
+  return Dual2(1.);
 }
 }  // namespace mdr

@@ -59,7 +59,7 @@ struct DevBuf {
   T *as() const { return static_cast<T *>(p); }
 };
 
-constexpr int kStatCount = 4;  // traces, fp64 rechecks, trough probes, unresolved traces
+constexpr int kStatCount = 4;  // traces, FP64 rechecks, trough/dip probes, FP64 rescans
 
 }  // namespace
 
@@ -112,6 +112,8 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, doubl
   if (int e = s->tr_q.ensure(qcap * sizeof(uint32_t))) return e;
   if (int e = s->counters.ensure(4 * sizeof(int))) return e;
   if (int e = s->overflow.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
+  if (int e = s->stats.ensure(kStatCount * sizeof(unsigned long long))) return e;
+  MD_HIP(hipMemsetAsync(s->stats.p, 0, kStatCount * sizeof(unsigned long long), s->stream));
   mdr::PipeArgs a{};
   a.traces = s->traces.as<mdr::TraceInfo>();
   a.models = s->models.as<mdr::ModelInfo>();
@@ -129,6 +131,8 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, doubl
   a.overflow_list = s->overflow.as<int>();
   a.queue_cap = static_cast<int>(qcap);
   a.nc = s->cfg.nc;
+  a.adaptive = s->cfg.grid;
+  a.rho = static_cast<float>(s->cfg.rho);
   a.kmax = kmax;
   a.wave = wave;
   a.fp32 = s->cfg.single_precision ? 1 : 0;
@@ -141,6 +145,7 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, doubl
   a.counts = counts;
   a.group = group;
   a.group_flags = gflags;
+  a.stats = s->stats.as<unsigned long long>();
   const int persistent = s->cus * 8;
   const int nchunks = static_cast<int>((ntraces + chunk - 1) / chunk);
   while (static_cast<int>(s->marks.size()) < nchunks * mdr::kStageMarks) {
@@ -161,7 +166,9 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, doubl
 }
 
 int check_config(const md_rocm_config &c) {
-  if (c.nc != 256 && c.nc != 512 && c.nc != 1024) return fail(2, "nc must be 256, 512 or 1024");
+  if (c.nc < 3 || c.nc > 65536) return fail(2, "nc must be 3..65536");
+  if (c.grid != 0 && c.grid != 1) return fail(2, "grid must be 0 (uniform) or 1 (adaptive)");
+  if (!(c.rho >= 1. && c.rho <= 1024.)) return fail(2, "rho must be 1..1024 nodes per mode");
   if (!(c.close_root_rtol > 0. && c.close_root_rtol < 1.)) return fail(2, "close_root_rtol must be in (0, 1)");
   if (!(c.group_rtol > 0. && c.group_rtol < 1.)) return fail(2, "group_rtol must be in (0, 1)");
   if (c.continuation && c.anchor_stride < 2) return fail(2, "anchor_stride must be at least 2");
@@ -187,6 +194,8 @@ void md_rocm_default_config(md_rocm_config *cfg) {
   cfg->tau_floor = 1e-4;
   cfg->tau_phase = 5e-5;
   cfg->chunk_traces = 0;
+  cfg->grid = 1;
+  cfg->rho = 8.;
 }
 
 const char *md_rocm_last_error(void) { return g_error.c_str(); }
@@ -363,7 +372,8 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
   }
   if (prediction_flags) std::fill_n(prediction_flags, ntr, 8u);  // bypassed: independent periods
   if (stats && nstats > 0) {
-    uint64_t st4[kStatCount] = {static_cast<uint64_t>(ntr), 0, 0, 0};
+    unsigned long long st4[kStatCount] = {0, 0, 0, 0};
+    MD_HIP(hipMemcpy(st4, s->stats.p, sizeof(st4), hipMemcpyDeviceToHost));
     for (int i = 0; i < std::min(nstats, kStatCount); ++i) stats[i] = st4[i];
   }
   return 0;

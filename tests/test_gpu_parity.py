@@ -43,21 +43,26 @@ def gpu_solve(md, h, vp, vs, rho, periods, bounds, nmodes, wave, scan="float32",
 
 
 def assert_parity(got, ref, group=True, label=""):
-    np.testing.assert_array_equal(got["root_count"], ref["root_count"], err_msg=f"{label} root counts")
+    """Traces the oracle itself leaves unresolved (count -1: its six-level
+    search never agreed) have no reference values and are not compared."""
+    resolved = ref["root_count"] >= 0
+    cells = np.broadcast_to(resolved[:, None, :], ref["phase_velocity"].shape)
+    np.testing.assert_array_equal(got["root_count"][resolved], ref["root_count"][resolved],
+                                  err_msg=f"{label} root counts")
     gv, rv = got["phase_velocity"], ref["phase_velocity"]
-    np.testing.assert_array_equal(np.isfinite(gv), np.isfinite(rv), err_msg=f"{label} validity mask")
+    np.testing.assert_array_equal(np.isfinite(gv)[cells], np.isfinite(rv)[cells], err_msg=f"{label} validity mask")
     ok = np.isfinite(rv)
     if ok.any():
         rel = np.abs(gv[ok] - rv[ok]) / rv[ok]
         assert rel.max() <= RTOL, f"{label} phase rel {rel.max():.3e}"
     if group:
         gg, rg = got["group_velocity"], ref["group_velocity"]
-        np.testing.assert_array_equal(np.isfinite(gg), np.isfinite(rg), err_msg=f"{label} group mask")
+        np.testing.assert_array_equal(np.isfinite(gg)[cells], np.isfinite(rg)[cells], err_msg=f"{label} group mask")
         gok = np.isfinite(rg)
         if gok.any():
             grel = np.abs(gg[gok] - rg[gok]) / np.abs(rg[gok])
             assert grel.max() <= RTOL, f"{label} group rel {grel.max():.3e}"
-        np.testing.assert_array_equal(got["group_velocity_flags"], ref["group_velocity_flags"],
+        np.testing.assert_array_equal(got["group_velocity_flags"][cells], ref["group_velocity_flags"][cells],
                                       err_msg=f"{label} group flags")
 
 
