@@ -32,7 +32,7 @@ SUPPORTS_CANONICAL_LOWER = True
 SUPPORTS_ROW_PERIODS = True
 SUPPORTS_GROUP_VELOCITY = True
 SUPPORTS_CONTINUATION = False
-STAT_NAMES = ("traces", "fp64_rechecks", "pair_probes", "fp64_rescans")
+STAT_NAMES = ("traces", "fp64_rechecks", "pair_probes", "fp64_rescans", "love_phase_searches", "scan_nodes")
 
 __all__ = ["load", "QeSolver", "available", "set_device", "memory_info", "device_name", "bounds",
            "BackendUnavailableError", "ABI_VERSION", "VERSION"]

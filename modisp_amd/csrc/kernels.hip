@@ -114,6 +114,22 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
   return v;
 }
 
+// The phase velocity whose wavenumber omega/c is exactly representable in
+// FP32: evaluating F in FP32 at k = float(omega / c) evaluates it at this c.
+// The result is kept inside [lo, hi] (one FP32 step of k inward).
+__device__ __forceinline__ double snap32(double omega, double c, double lo, double hi) {
+  float k = float(omega / c);
+  double s = omega / double(k);
+  if (s > hi) {
+    k = __int_as_float(__float_as_int(k) + 1);  // k > 0: next float up
+    s = omega / double(k);
+  } else if (s < lo) {
+    k = __int_as_float(__float_as_int(k) - 1);  // next float down
+    s = omega / double(k);
+  }
+  return s;
+}
+
 // Secular F with layer constants formed on the fly from global rows (models
 // too deep for an LDS layer table).
 template <class T>
@@ -155,20 +171,21 @@ __device__ __forceinline__ T secular_global(const PrepRow *rows, int L, int top,
 // K-th certain sign change, when no later node can change the first K roots.
 // ---------------------------------------------------------------------------
 constexpr int kWaves = kBlock / 64;
-constexpr int kTabMax = kFloor + 2 * kAdaptRows;
 
-template <class T>
+// Per-wavefront LDS scratch for models of at most R rows.
+template <class T, int R>
 struct WaveScratch {
-  Layer<T> lay[kAdaptRows];
-  double tc[kTabMax];        // table coordinates, ascending
-  double ks[2 * kAdaptRows];  // layer speeds inside the window (unsorted)
-  float tn[kTabMax];         // N at table points
-  int pre[kTabMax];          // node index of table point m
-  unsigned char kink[kTabMax];
+  static constexpr int kTab = kFloor + 2 * R;
+  Layer<T> lay[R];
+  double tc[kTab];   // table coordinates, ascending
+  double ks[2 * R];  // layer speeds inside the window, in row order
+  float tn[kTab];    // N at table points
+  int pre[kTab];     // node index of table point m
+  unsigned char kink[kTab];
 };
 
-template <class T>
-__device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane) {
+template <class T, int R>
+__device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int lane) {
   const TraceInfo ti = a.traces[t];
   const ModelInfo mi = a.models[ti.model];
   const int tl = t - a.trace_begin;
@@ -189,39 +206,41 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane
     }
     return;
   }
-  const int L = mi.L, top = mi.top, K = mi.K;
+  const int L = mi.L, top = mi.top, K = mi.K;  // L <= R (the launcher picks R)
   const double omega = ti.omega, iw2 = 1. / (omega * omega);
   const PrepRow *rows = a.rows + mi.row0;
-  const bool in_lds = L <= kAdaptRows;
-  if (in_lds)
-    for (int j = lane; j < L; j += 64) sh.lay[j] = make_layer<T>(rows[j], omega, iw2, a.wave);
+  for (int j = lane; j < L; j += 64) sh.lay[j] = make_layer<T>(rows[j], omega, iw2, a.wave);
   wave_sync();
-  const bool adaptive = a.adaptive && in_lds;
+  const bool adaptive = a.adaptive;
   int total = a.nc, ntab = 0;
   const double dc = (hi - lo) / double(a.nc - 1);
   if (adaptive) {
-    // Layer speeds strictly inside the window (row `lane`; L <= 64).
-    double v0 = 0., v1 = 0.;
-    int nmine = 0;
-    if (lane < L - 1 && rows[lane].h > 0.) {
-      const PrepRow r = rows[lane];
-      if (r.ivs > 0.) {
-        const double v = 1. / r.ivs;
-        if (v > lo && v < hi) { v0 = v; ++nmine; }
-      }
-      if (a.wave == 0) {
-        const double v = 1. / r.ivp;
-        if (v > lo && v < hi) {
-          if (nmine) v1 = v; else v0 = v;
-          ++nmine;
+    // Layer speeds strictly inside the window, compacted in row order.
+    int nk = 0;
+    for (int base = 0; base < L - 1; base += 64) {
+      const int j = base + lane;
+      double v0 = 0., v1 = 0.;
+      int nmine = 0;
+      if (j < L - 1 && rows[j].h > 0.) {
+        const PrepRow r = rows[j];
+        if (r.ivs > 0.) {
+          const double v = 1. / r.ivs;
+          if (v > lo && v < hi) { v0 = v; ++nmine; }
+        }
+        if (a.wave == 0) {
+          const double v = 1. / r.ivp;
+          if (v > lo && v < hi) {
+            if (nmine) v1 = v; else v0 = v;
+            ++nmine;
+          }
         }
       }
+      const int incl = wave_incl_scan(nmine, lane);
+      const int off = nk + incl - nmine;
+      if (nmine > 0) sh.ks[off] = v0;
+      if (nmine > 1) sh.ks[off + 1] = v1;
+      nk += __shfl(incl, 63, 64);
     }
-    const int incl = wave_incl_scan(nmine, lane);
-    const int nk = __shfl(incl, 63, 64);
-    const int off = incl - nmine;
-    if (nmine > 0) sh.ks[off] = v0;
-    if (nmine > 1) sh.ks[off + 1] = v1;
     wave_sync();
     const double dt = (hi - lo) / double(kFloor - 1);
     auto floor_point = [&](int m) { return m == kFloor - 1 ? hi : lo + dt * double(m); };
@@ -240,7 +259,8 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane
       }
       return n * 0.3183098861837907f;  // 1/pi
     };
-    // Merge the kFloor uniform points with the layer speeds by rank.
+    // Merge the kFloor (= 64, one per lane) uniform points with the layer
+    // speeds by rank.
     {
       const double tm = floor_point(lane);
       int below = 0;
@@ -249,13 +269,12 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane
       sh.kink[lane + below] = 0;
       sh.tn[lane + below] = density(tm);
     }
-    for (int q = 0; q < nmine; ++q) {
-      const double v = q ? v1 : v0;
-      const int me = off + q;
+    for (int q = lane; q < nk; q += 64) {
+      const double v = sh.ks[q];
       int rank = 0;
       for (int r = 0; r < nk; ++r) {
         const double u = sh.ks[r];
-        rank += (u < v) || (u == v && r < me);
+        rank += (u < v) || (u == v && r < q);
       }
       int ubelow = 0;
       for (int m = 0; m < kFloor; ++m) ubelow += floor_point(m) <= v;
@@ -308,14 +327,15 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane
     bool u = false;
     if (active) {
       c = node_c(j);
+      // FP32: an interior node moves to the phase velocity whose wavenumber is
+      // the FP32 k actually evaluated, so a node's sign belongs to its
+      // coordinate. The window ends stay exact and are always rechecked in FP64.
+      const bool end = j == 0 || j == total - 1;
+      if (fp32 && !end) c = snap32(omega, c, lo, hi);
       const T k = T(omega / c);
-      T F;
-      if (in_lds)
-        F = a.wave == 0 ? rayleigh_F_scale<T>(sh.lay, L, top, k, &sc) : love_F_scale<T>(sh.lay, L, top, k, &sc);
-      else
-        F = secular_global<T>(rows, L, top, a.wave, omega, iw2, k, &sc);
+      const T F = a.wave == 0 ? rayleigh_F_scale<T>(sh.lay, L, top, k, &sc) : love_F_scale<T>(sh.lay, L, top, k, &sc);
       f = float(F);
-      u = fp32 && (!(fabsf(f) >= tau) || c >= hi_guard);
+      u = fp32 && (!(fabsf(f) >= tau) || c >= hi_guard || end);
     }
     const float fl = __shfl(f, (lane + 63) & 63, 64), fr = __shfl(f, (lane + 1) & 63, 64);
     const float sl = __shfl(sc, (lane + 63) & 63, 64), sr = __shfl(sc, (lane + 1) & 63, 64);
@@ -325,18 +345,25 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane
     const bool has_left = j >= 1, has_right = j + 1 < total;
     // Lanes 1..62 see both neighbours; lane 0 only at the first node.
     const bool classify = active && (lane >= 1 || base == 0) && lane <= 62;
-    int kind = kNone;
+    int kind = kNone, probe = 0;
     if (classify) {
       if (u) kind = kUncertain;
       else if (has_right && !ur && signbit(f) != signbit(fr)) kind = kBracket;
-      else if (has_left && has_right && !ul && !ur && signbit(fl) == signbit(f) && signbit(fr) == signbit(f)) {
-        // Same-sign window: a hidden root pair shows as a small local minimum
-        // of |F| (trough) or as a dip of the log2 normalization scale (a
-        // collapsing state vector between two very close roots).
+      if (has_left && has_right) {
+        // A hidden root pair in a same-sign cell shows as a small local
+        // minimum of |F| (trough) or as a dip of the log2 normalization scale
+        // (a state vector that collapses between two very close roots). The
+        // scale needs no sign, so a dip is flagged next to uncertain nodes too;
+        // the probe later searches only the cells whose end signs agree.
         const float af = fabsf(f);
-        if (af < fabsf(fl) && af < fabsf(fr) && af < theta) kind = kTrough;
-        else if (sc < sl && sc < sr && 0.5f * (sl + sr) - sc > kDipDepth) kind = kDip;
+        const bool certain3 = !u && !ul && !ur;
+        const bool same_l = signbit(fl) == signbit(f), same_r = signbit(fr) == signbit(f);
+        if (certain3 && same_l && same_r && af < fabsf(fl) && af < fabsf(fr) && af < theta) probe = kTrough;
+        else if (sc < sl && sc < sr && 0.5f * (sl + sr) - sc > kDipDepth && !(certain3 && !same_l && !same_r) &&
+                 (a.wave == 0 || (certain3 && same_l && same_r)))
+          probe = kDip;  // Love: two-sided only; its exact phase count catches the rest
       }
+      if (probe && kind == kNone) kind = kProbe;
     }
     const uint64_t bm = __ballot(kind == kBracket);
     uint64_t em = __ballot(kind != kNone);
@@ -356,7 +383,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane
       break;
     }
     const bool mine = (em >> lane) & 1;
-    const uint64_t uqm = em & __ballot(kind == kUncertain), tqm = em & __ballot(kind == kTrough || kind == kDip);
+    const uint64_t uqm = em & __ballot(kind == kUncertain), tqm = em & __ballot(probe != 0);
     int qu = 0, qt = 0;
     if (lane == 0) {
       if (uqm) qu = atomicAdd(a.unc_count, __popcll(uqm));
@@ -372,16 +399,21 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane
       x.c2 = has_right ? cr : NAN;
       x.v = 0.;
       x.x = NAN;
+      x.vx = 0.;
       x.fl = fl;
       x.fc = f;
       x.fr = fr;
+      x.sl = sl;
+      x.sc = sc;
+      x.sr = sr;
       x.node = j;
       x.kind = kind;
-      x.sc = sc;
+      x.probe = probe;
+      x.pad = 0;
       ev[slot] = x;
       const uint32_t g = static_cast<uint32_t>(tl * kEventCap + slot);
       if (kind == kUncertain) a.unc_queue[qu + __popcll(uqm & lanes_below(lane))] = g;
-      if (kind == kTrough || kind == kDip) a.trough_queue[qt + __popcll(tqm & lanes_below(lane))] = g;
+      if (probe) a.trough_queue[qt + __popcll(tqm & lanes_below(lane))] = g;
     }
     ne += nnew;
     if (done) break;
@@ -394,22 +426,26 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T> &sh, int lane
   }
 }
 
-template <class T>
-__global__ __launch_bounds__(kBlock) void scan_kernel(PipeArgs a) {
-  __shared__ WaveScratch<T> scratch[kWaves];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int t = a.trace_begin + blockIdx.x * kWaves + w;
-  if (t < a.trace_end) scan_wave<T>(a, t, scratch[w], lane);
+// R: rows the LDS scratch holds; W: wavefronts (traces) per block.
+template <class T, int R, int W>
+__global__ __launch_bounds__(64 * W) void scan_kernel(PipeArgs a) {
+  __shared__ WaveScratch<T, R> scratch[W];
+  // The wave index is uniform across the wavefront; readfirstlane lets the
+  // compiler keep every per-trace quantity in scalar registers.
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int t = a.trace_begin + blockIdx.x * W + w;
+  if (t < a.trace_end) scan_wave<T, R>(a, t, scratch[w], lane);
 }
 
 // FP64 rescan of the traces whose FP32 event list overflowed (persistent,
 // one wavefront per listed trace).
-__global__ __launch_bounds__(kBlock) void rescan_kernel(PipeArgs a) {
-  __shared__ WaveScratch<double> scratch[kWaves];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+template <int R, int W>
+__global__ __launch_bounds__(64 * W) void rescan_kernel(PipeArgs a) {
+  __shared__ WaveScratch<double, R> scratch[W];
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int n = *a.overflow_count;
-  for (int q = blockIdx.x * kWaves + w; q < n; q += gridDim.x * kWaves)
-    scan_wave<double>(a, a.overflow_list[q] + a.trace_begin, scratch[w], lane);
+  for (int q = blockIdx.x * W + w; q < n; q += gridDim.x * W)
+    scan_wave<double, R>(a, __builtin_amdgcn_readfirstlane(a.overflow_list[q]) + a.trace_begin, scratch[w], lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -428,6 +464,46 @@ __global__ __launch_bounds__(kBlock) void recheck_kernel(PipeArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Best available value at the three nodes of event e: FP64 where a node was
+// rechecked (an uncertain neighbour is always the adjacent event), else the scan's.
+// ---------------------------------------------------------------------------
+struct NodeValues {
+  double l, c, r;
+  bool lu, ru;
+};
+__device__ inline NodeValues node_values(const Event *ev, int e, int nev) {
+  const Event &x = ev[e];
+  NodeValues n;
+  n.lu = e > 0 && ev[e - 1].node == x.node - 1 && ev[e - 1].kind == kUncertain;
+  n.ru = e + 1 < nev && ev[e + 1].node == x.node + 1 && ev[e + 1].kind == kUncertain;
+  n.l = n.lu ? ev[e - 1].v : double(x.fl);
+  n.r = n.ru ? ev[e + 1].v : double(x.fr);
+  n.c = x.kind == kUncertain ? x.v : double(x.fc);
+  return n;
+}
+
+// Trough test at rechecked nodes: once an uncertain node has its FP64 value,
+// a same-sign local minimum of |F| is probed like a certain one, so that an
+// FP32-uncertain node never masks a pair of close roots. (Dips need no sign
+// and are flagged by the scan itself.)
+__global__ __launch_bounds__(kBlock) void unc_probe_kernel(PipeArgs a) {
+  const int n = min(*a.unc_count, a.queue_cap);
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+    const uint32_t g = a.unc_queue[q];
+    const int tl = g / kEventCap, e = g % kEventCap;
+    const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
+    const Event &x = ev[e];
+    if (x.probe || !(x.c0 == x.c0) || !(x.c2 == x.c2)) continue;
+    const NodeValues v = node_values(ev, e, a.scan[tl].nev);
+    if (signbit(v.l) != signbit(v.c) || signbit(v.r) != signbit(v.c)) continue;
+    if (!(fabs(v.c) < fabs(v.l) && fabs(v.c) < fabs(v.r) && fabs(v.c) < double(kTroughMax))) continue;
+    a.events[g].probe = kTrough;
+    const int slot = atomicAdd(a.trough_count, 1);
+    if (slot < a.queue_cap) a.trough_queue[slot] = g;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 probe of same-sign troughs: golden-section search for the minimum of
 // sign*F over [c_{i-1}, c_{i+1}], stopping at the first opposite-sign point.
 // ---------------------------------------------------------------------------
@@ -437,17 +513,37 @@ __global__ __launch_bounds__(kBlock) void trough_kernel(PipeArgs a) {
     const uint32_t g = a.trough_queue[q];
     const int tl = g / kEventCap;
     Event &x = a.events[g];
-    if (x.kind != kTrough && x.kind != kDip) continue;
-    const bool dip = x.kind == kDip;
+    if (x.probe != kTrough && x.probe != kDip) continue;
+    const bool dip = x.probe == kDip;
+    // Search only the neighbouring cells whose end values share the node's sign.
+    const NodeValues nv = node_values(a.events + static_cast<size_t>(tl) * kEventCap, g % kEventCap, a.scan[tl].nev);
+    const bool left = x.c0 == x.c0 && signbit(nv.l) == signbit(nv.c);
+    const bool right = x.c2 == x.c2 && signbit(nv.r) == signbit(nv.c);
+    if (!left && !right) continue;
     const TraceInfo ti = a.traces[tl + a.trace_begin];
     const ModelInfo mi = a.models[ti.model];
     const PrepRow *rows = a.rows + mi.row0;
-    double lo = x.c0, hi = x.c2;
-    const double sign = x.fc < 0.f ? -1. : 1.;
+    double lo = left ? x.c0 : x.c1, hi = right ? x.c2 : x.c1;
+    const double sign = signbit(nv.c) ? -1. : 1.;
     // Golden-section minimization of sign*F (trough) or of the log2 scale
     // (dip), stopping at the first point where F takes the opposite sign.
-    auto probe = [&](double c, float &key) {
+    // FP32 path: probe points are evaluated in FP32 and rechecked in FP64 only
+    // inside the FP32 uncertainty band (or the cutoff guard), as in the scan.
+    const float tau = ti.tau;
+    const double iw2 = 1. / (ti.omega * ti.omega);
+    auto probe = [&](double &c, float &key) {
       float ls;
+      // (FP32 only while the interval is far wider than the FP32 spacing of k.)
+      if (a.fp32 && c < mi.cutoff_guard && hi - lo > 1e-5 * hi) {
+        const double cs = snap32(ti.omega, c, lo, hi);
+        const float f32 = secular_global<float>(rows, mi.L, mi.top, a.wave, ti.omega, iw2, float(ti.omega / cs), &ls);
+        if (fabsf(f32) >= tau && cs > lo && cs < hi) {
+          c = cs;  // the point actually evaluated
+          const double f = sign * double(f32);
+          key = dip ? ls : float(f);
+          return f;
+        }
+      }
       const double f = sign * secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, c, &ls);
       key = dip ? ls : float(f);
       return f;
@@ -476,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void trough_kernel(PipeArgs a) {
       }
     }
     x.x = split;
-    x.v = fsplit;
+    x.vx = fsplit;
   }
 }
 
@@ -497,39 +593,158 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(PipeArgs a) {
   if (ts.status == 0 && !(flags & kTraceOverflow)) {
     const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
     auto emit = [&](double lo, double hi, double flo, double fhi) {
-      if (nb < a.kmax) out[nb] = Bracket{lo, hi, flo, fhi};
+      if (nb < K) out[nb] = Bracket{lo, hi, flo, fhi};
       ++nb;
     };
-    int prev_unc = -2;
-    double prev_v = 0.;
+    int last_split = -2;  // left node of the cell that holds the last split pair
+    // Both roots of a split pair lie in the same-sign cell [lo, hi] that holds the split.
+    auto emit_pair = [&](const Event &x, int cell, double lo, double hi, double flo, double fhi) {
+      if (cell == last_split || !(x.x > lo && x.x < hi) || signbit(flo) != signbit(fhi) ||
+          signbit(x.vx) == signbit(flo))
+        return;
+      last_split = cell;
+      emit(lo, x.x, flo, x.vx);
+      if (nb < K) emit(x.x, hi, x.vx, fhi);
+    };
     for (int e = 0; e < ts.nev && nb < K; ++e) {
       const Event x = ev[e];
-      if (x.kind == kBracket) {
-        emit(x.c1, x.c2, x.fc, x.fr);
-      } else if (x.kind == kTrough || x.kind == kDip) {
-        if (x.x == x.x) {  // hidden pair split at x
-          emit(x.c0, x.x, x.fl, x.v);
-          if (nb < K) emit(x.x, x.c2, x.v, x.fr);
-        }
-      } else {  // uncertain node with its FP64 value; owns its left cell when
-                // that neighbour is certain and always its right cell
-        const double v = x.v;
-        if (x.c0 == x.c0) {
-          const double fl = prev_unc == x.node - 1 ? prev_v : double(x.fl);
-          if (signbit(fl) != signbit(v) && nb < K) emit(x.c0, x.c1, fl, v);
-        }
-        const bool right_unc = e + 1 < ts.nev && ev[e + 1].kind == kUncertain && ev[e + 1].node == x.node + 1;
-        if (!right_unc && x.c2 == x.c2 && nb < K) {
-          const double fr = double(x.fr);
-          if (signbit(v) != signbit(fr)) emit(x.c1, x.c2, v, fr);
-        }
-        prev_unc = x.node;
-        prev_v = v;
-      }
+      const NodeValues v = node_values(ev, e, ts.nev);
+      const bool split = x.probe && x.x == x.x, split_left = split && x.x < x.c1;
+      // Left cell [c0, c1]: an uncertain node owns it when that neighbour is certain.
+      if (split_left) emit_pair(x, x.node - 1, x.c0, x.c1, v.l, v.c);
+      else if (x.kind == kUncertain && !v.lu && x.c0 == x.c0 && signbit(v.l) != signbit(v.c))
+        emit(x.c0, x.c1, v.l, v.c);
+      if (nb >= K) break;
+      // Right cell [c1, c2]: a certain sign change, or an uncertain node's.
+      if (split && !split_left) emit_pair(x, x.node, x.c1, x.c2, v.c, v.r);
+      else if (x.kind == kBracket || (x.kind == kUncertain && x.c2 == x.c2 && signbit(v.c) != signbit(v.r)))
+        emit(x.c1, x.c2, v.c, v.r);
     }
   }
   const int count = nb < K ? nb : K;
   ts.count = (flags & kTraceOverflow) ? -1 : (ts.status ? 0 : count);
+}
+
+// ---------------------------------------------------------------------------
+// Love waves: exact root count from the unwrapped SH phase (reference
+// cpp/common/love_phase.hpp:9-36). Roots are the c where the phase crosses
+// n*pi; the phase decreases with c.
+// ---------------------------------------------------------------------------
+__device__ double love_phase_dev(const PrepRow *rows, int L, int top, double omega, double c) {
+  constexpr double pi = 3.141592653589793;
+  const int last = L - 1;
+  const double k = omega / c, khalf = omega * rows[last].ivs;
+  if (!(k > khalf)) return NAN;
+  double scale = 0.5 * rows[last].rho * rows[last].g2 * sqrt((k + khalf) * (k - khalf));
+  double theta = .25 * pi;
+  auto unwrap = [](double raw, double prev) { return prev + atan2(sin(raw - prev), cos(raw - prev)); };
+  for (int j = last - 1; j >= top; --j) {
+    const PrepRow r = rows[j];
+    if (r.h == 0.) continue;
+    const double mu = 0.5 * r.rho * r.g2, kb = omega * r.ivs;
+    const double q = sqrt(fabs((k + kb) * (k - kb))), next = mu * (q > 0. ? q : 1. / r.h);
+    if (!(scale > 0.) || !(next > 0.) || !(fabs(scale / next) < INFINITY)) return NAN;
+    double sn, cs;
+    sincos(theta, &sn, &cs);
+    theta = unwrap(atan2((scale / next) * sn, cs), theta);
+    scale = next;
+    if (k < kb) theta -= q * r.h;
+    else {
+      sincos(theta, &sn, &cs);
+      double t, u;
+      if (q == 0.) { t = sn; u = cs + sn * r.h * scale / mu; }
+      else {
+        const double ex = exp(-2. * q * r.h), ch = .5 * (1. + ex), sh = .5 * (1. - ex);
+        t = sn * ch + cs * sh;
+        u = cs * ch + sn * sh;
+      }
+      if (fmax(fabs(t), fabs(u)) < 1e-14) return NAN;
+      theta = unwrap(atan2(t, u), theta);
+    }
+    if (!(fabs(theta) <= 1048576.)) return NAN;
+  }
+  return theta;
+}
+
+// Number of roots in [lo, c]: multiples of pi in (phase(c), phase(lo)].
+__device__ inline int love_roots_between(double plo, double pc) {
+  constexpr double pi = 3.141592653589793;
+  const double top = floor(plo / pi);
+  const double n = top - floor(pc / pi);
+  return n > 0. ? int(n) : 0;
+}
+
+// One lane per Love trace: compares the assembled brackets with the exact
+// phase count and queues the trace for a phase search when they disagree.
+__global__ __launch_bounds__(kBlock) void love_count_kernel(PipeArgs a) {
+  const int tl = blockIdx.x * kBlock + threadIdx.x;
+  const int t = tl + a.trace_begin;
+  if (t >= a.trace_end) return;
+  TraceScan &ts = a.scan[tl];
+  if (ts.status || (ts.flags & kTraceOverflow)) return;
+  const TraceInfo ti = a.traces[t];
+  const ModelInfo mi = a.models[ti.model];
+  const PrepRow *rows = a.rows + mi.row0;
+  const double pl = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.lo);
+  const double ph = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.hi);
+  if (!(pl == pl) || !(ph == ph) || ph > pl) {
+    ts.flags |= kFlagLoveFallback;
+    return;
+  }
+  const int exact = love_roots_between(pl, ph), K = mi.K;
+  const int want = exact < K ? exact : K;
+  bool ok = ts.count == want;
+  if (ok && want == K && exact > K) {
+    // Capped: the K-th bracket must end below the (K+1)-th phase crossing.
+    const Bracket b = a.brackets[static_cast<size_t>(tl) * a.kmax + K - 1];
+    const double pb = love_phase_dev(rows, mi.L, mi.top, ti.omega, b.hi);
+    ok = pb == pb && love_roots_between(pl, pb) == K;
+  }
+  if (ok) {
+    ts.flags |= kScanLovePhase;
+    return;
+  }
+  ts.count = 0;
+  ts.flags |= kScanLovePhase;
+  a.phase_list[atomicAdd(a.phase_count, 1)] = tl;
+}
+
+// Phase search of the queued Love traces (reference love_phase.hpp:48-91),
+// one lane per (trace, mode slot): bisection on the phase until the bracket
+// holds a single crossing, n*pi, with opposite signs of F at its ends.
+__global__ __launch_bounds__(kBlock) void love_phase_kernel(PipeArgs a) {
+  constexpr double pi = 3.141592653589793;
+  const int n = *a.phase_count;
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < n * a.kmax; q += gridDim.x * kBlock) {
+    const int tl = a.phase_list[q / a.kmax], m = q % a.kmax;
+    const TraceInfo ti = a.traces[tl + a.trace_begin];
+    const ModelInfo mi = a.models[ti.model];
+    if (m >= mi.K) continue;
+    const PrepRow *rows = a.rows + mi.row0;
+    double lo = mi.lo, hi = mi.hi;
+    double pa = love_phase_dev(rows, mi.L, mi.top, ti.omega, lo), pb = love_phase_dev(rows, mi.L, mi.top, ti.omega, hi);
+    const int exact = love_roots_between(pa, pb), want = exact < mi.K ? exact : mi.K;
+    if (m == 0) a.scan[tl].count = want;
+    if (m >= want) continue;
+    const double level = (floor(pa / pi) - double(m)) * pi;
+    Bracket br{NAN, NAN, 0., 0.};
+    double fa = 0., fb = 0.;
+    bool have_f = false;
+    for (int it = 0; it < 200; ++it) {
+      if (pa < level + pi && pb > level - pi) {
+        fa = secular_rows(rows, mi.L, mi.top, 1, ti.omega, lo);
+        fb = secular_rows(rows, mi.L, mi.top, 1, ti.omega, hi);
+        if (signbit(fa) != signbit(fb)) { have_f = true; break; }
+      }
+      const double mid = lo + .5 * (hi - lo);
+      if (!(mid > lo && mid < hi)) break;
+      const double p = love_phase_dev(rows, mi.L, mi.top, ti.omega, mid);
+      if (!(p == p)) break;
+      if (p > level) { lo = mid; pa = p; } else { hi = mid; pb = p; }
+    }
+    if (have_f) br = Bracket{lo, hi, fa, fb};
+    a.brackets[static_cast<size_t>(tl) * a.kmax + m] = br;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -586,7 +801,11 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
     if (m < ts.count && m < mi.K) {
       const Bracket br = a.brackets[q];
       int ev = 0;
-      root = brent(a.rows + mi.row0, mi.L, mi.top, a.wave, ti.omega, br.lo, br.hi, br.flo, br.fhi, a.tol, ev);
+      // Guard roots beyond the requested modes only feed the close-root test.
+      const double tol = m < mi.modes ? a.tol : fmax(a.tol, kGuardTol);
+      if (br.lo == br.lo) {
+        root = brent(a.rows + mi.row0, mi.L, mi.top, a.wave, ti.omega, br.lo, br.hi, br.flo, br.fhi, tol, ev);
+      }
     }
     a.roots[q] = root;
   }
@@ -606,6 +825,8 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(PipeArgs a) {
   unsigned flags = 0;
   int count = ts.count, level = a.fp32 ? kLevelScan32 : kLevelScan64;
   if (ts.status) { level = ts.status == 1 ? -4 : -3; count = 0; }
+  if (ts.flags & kScanLovePhase) level = kLevelLovePhase;
+  flags |= ts.flags & kFlagLoveFallback;
   if (ts.flags & kTraceOverflow) { flags |= kFlagUnconverged; level = -1; count = -1; }
   bool cutoff = false, close = false;
   int nfinite = 0;
@@ -705,13 +926,30 @@ __global__ void reset_kernel(int *counters, int n) {
 }
 
 // Accumulate this chunk's work counters into the run statistics:
-// [0] traces, [1] FP64 rechecks, [2] trough/dip probes, [3] FP64 rescans.
+// [0] traces, [1] FP64 rechecks, [2] pair probes, [3] FP64 rescans,
+// [4] Love phase searches, [5] evaluated scan nodes.
 __global__ void stats_kernel(const int *counters, unsigned long long *stats, int traces) {
   if (threadIdx.x == 0) {
     stats[0] += static_cast<unsigned long long>(traces);
     stats[1] += static_cast<unsigned long long>(counters[0]);
     stats[2] += static_cast<unsigned long long>(counters[1]);
     stats[3] += static_cast<unsigned long long>(counters[2]);
+    stats[4] += static_cast<unsigned long long>(counters[3]);
+  }
+}
+
+// Sum of the grid nodes every trace of the chunk evaluated (one atomic per block).
+__global__ __launch_bounds__(kBlock) void nodes_kernel(const TraceScan *scan, int n, unsigned long long *stats) {
+  unsigned long long s = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += scan[i].nodes;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  __shared__ unsigned long long part[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+    atomicAdd(stats + 5, t);
   }
 }
 
@@ -722,11 +960,16 @@ __global__ void stats_kernel(const int *counters, unsigned long long *stats, int
 // ---------------------------------------------------------------------------
 namespace mdr {
 
+// Models of up to kAdaptRows rows: four traces per block; deeper models (up to
+// kMaxRows) take the large-scratch variant, one trace per block.
 template <class T>
 static hipError_t launch_scan_t(const PipeArgs &a, hipStream_t s) {
   const int n = a.trace_end - a.trace_begin;
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL((scan_kernel<T>), dim3((n + kWaves - 1) / kWaves), dim3(kBlock), 0, s, a);
+  if (a.max_rows <= kAdaptRows)
+    hipLaunchKernelGGL((scan_kernel<T, kAdaptRows, kWaves>), dim3((n + kWaves - 1) / kWaves), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((scan_kernel<T, kMaxRows, 1>), dim3(n), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
@@ -743,19 +986,33 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
   if (err != hipSuccess) return err;
   mark(1);
   const int pb = persistent_blocks;
-  if (fp32) hipLaunchKernelGGL(rescan_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+  if (fp32) {
+    if (pa.max_rows <= kAdaptRows)
+      hipLaunchKernelGGL((rescan_kernel<kAdaptRows, kWaves>), dim3(pb), dim3(kBlock), 0, s, pa);
+    else
+      hipLaunchKernelGGL((rescan_kernel<kMaxRows, 1>), dim3(pb), dim3(64), 0, s, pa);
+  }
   mark(2);
   hipLaunchKernelGGL(recheck_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+  if (fp32) hipLaunchKernelGGL(unc_probe_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   hipLaunchKernelGGL(trough_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   mark(3);
   hipLaunchKernelGGL(assemble_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
+  if (pa.wave == 1) {
+    hipLaunchKernelGGL(love_count_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
+    hipLaunchKernelGGL(love_phase_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+  }
   mark(4);
   hipLaunchKernelGGL(refine_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   hipLaunchKernelGGL(finalize_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   mark(5);
   if (group) hipLaunchKernelGGL(group_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   mark(6);
-  if (pa.stats) hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(64), 0, s, pa.counters, pa.stats, n);
+  if (pa.stats) {
+    hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(64), 0, s, pa.counters, pa.stats, n);
+    hipLaunchKernelGGL(nodes_kernel, dim3((n + kBlock - 1) / kBlock < 1024 ? (n + kBlock - 1) / kBlock : 1024), dim3(kBlock), 0, s, pa.scan, n,
+                       pa.stats);
+  }
   return hipGetLastError();
 }
 

@@ -20,24 +20,28 @@ constexpr int kBlock = 256;       // lanes per scan block (4 wave64)
 constexpr int kMaxRows = 256;     // layer rows per model (reference limit, __init__.py:152)
 constexpr int kEventCap = 32;     // ascending events kept per trace
 constexpr int kFloor = 64;        // uniform table points of the adaptive grid (one per lane)
-constexpr int kAdaptRows = 64;    // rows held in an LDS layer table / eligible for the adaptive grid
+constexpr int kAdaptRows = 64;    // rows of the common scan scratch; deeper models use a kMaxRows scratch
 constexpr int kCellMax = 1 << 16; // density nodes per table cell (cap)
 constexpr float kTroughMax = 0.05f;  // |F| below which a same-sign trough is probed
 constexpr int kTroughIters = 28;  // golden-section steps per trough / dip probe
 constexpr float kDipDepth = 0.5f; // log2-scale dip (bits below the neighbours' mean) that is probed
+constexpr double kGuardTol = 1e-7;  // relative tolerance of the guard roots (modes+1, modes+2)
 constexpr int kDipStopIter = 6;   // a dip probe gives up after this many steps ...
 constexpr float kDipStopDrop = 1.5f;  // ... unless the log2 scale fell this far below the node's
 
-enum EventKind : int { kNone = 0, kBracket = 1, kUncertain = 2, kTrough = 3, kDip = 4 };
+// Node kinds of a scan event (kBracket, kUncertain, kProbe) and probe kinds
+// (kTrough, kDip) of a node searched for a hidden root pair.
+enum EventKind : int { kNone = 0, kBracket = 1, kUncertain = 2, kTrough = 3, kDip = 4, kProbe = 5 };
 
 // Trace-flag bits (reference python/modisp/__init__.py:15-32).
 constexpr unsigned kFlagUnconverged = 4u, kFlagNearCutoff = 16u, kFlagRefineFailed = 64u,
-                   kFlagCloseRoot = 4096u;
+                   kFlagCloseRoot = 4096u, kFlagLoveFallback = 16384u;
 constexpr unsigned kTraceOverflow = 1u << 31;  // internal: event list overflowed
 // Group-velocity flags (reference cpp/common/group_velocity.hpp:9-12).
 constexpr unsigned kGroupMissing = 1u, kGroupClose = 16u, kGroupNonfinite = 32u;
 // search_level codes of this backend's grid scans.
-constexpr int kLevelScan32 = 20, kLevelScan64 = 21;
+constexpr int kLevelScan32 = 20, kLevelScan64 = 21, kLevelLovePhase = 11;
+constexpr unsigned kScanLovePhase = 1u << 30;  // TraceScan: Love roots follow the exact phase count
 
 // One prepared layer row: reciprocals so a trace's constants are products.
 struct PrepRow {
@@ -79,12 +83,15 @@ struct TraceInfo {
 
 struct Event {
   double c0, c1, c2;  // nodes j-1, j, j+1 (NaN where absent); a bracket is [c1, c2]
-  double v;           // FP64 value at c1 (uncertain) / F at the split (trough)
-  double x;           // trough split point (NaN: none)
+  double v;           // FP64 value at c1 (uncertain nodes)
+  double x;           // split point of a hidden root pair (NaN: none)
+  double vx;          // F at the split point
   float fl, fc, fr;   // scan values at c0, c1, c2
+  float sl, sc, sr;   // summed log2 normalization scales at c0, c1, c2
   int node;           // node index j within the trace
-  int kind;
-  float sc;           // summed log2 normalization scale at c1
+  int kind;           // EventKind of the scan classification
+  int probe;          // kTrough / kDip when the node is probed for a hidden pair
+  int pad;
 };
 
 struct TraceScan {
@@ -110,14 +117,17 @@ struct PipeArgs {
   double *roots;
   uint32_t *unc_queue;
   uint32_t *trough_queue;
-  int *counters;  // [0] uncertain nodes, [1] troughs, [2] overflowed FP32 traces
+  int *counters;  // [0] uncertain nodes, [1] pair probes, [2] overflowed FP32 traces, [3] Love phase searches
   int *unc_count;
   int *trough_count;
   int *overflow_count;
   int *overflow_list;  // chunk-local trace indices to rescan in FP64
+  int *phase_count;
+  int *phase_list;     // chunk-local Love traces whose brackets disagree with the phase count
   int queue_cap;
   int trace_begin, trace_end;
   int nc;
+  int max_rows;   // largest row count of any model in the batch (<= kMaxRows)
   int adaptive;   // 1: density-adaptive grid, 0: uniform nc-node grid
   float rho;      // adaptive grid: nodes per WKB mode
   int kmax;

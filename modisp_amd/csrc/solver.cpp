@@ -59,7 +59,7 @@ struct DevBuf {
   T *as() const { return static_cast<T *>(p); }
 };
 
-constexpr int kStatCount = 4;  // traces, FP64 rechecks, trough/dip probes, FP64 rescans
+constexpr int kStatCount = 6;  // traces, FP64 rechecks, pair probes, FP64 rescans, Love phase searches, scan nodes
 
 }  // namespace
 
@@ -70,7 +70,7 @@ struct md_rocm_solver {
   int cus = 256;
   hipStream_t stream = nullptr;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats, overflow;
+  DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats, overflow, phase;
   DevBuf in_h, in_vp, in_vs, in_rho, in_per, in_bounds;
   DevBuf out_v, out_flags, out_levels, out_counts, out_group, out_gflags;
   double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -79,7 +79,7 @@ struct md_rocm_solver {
 
   ~md_rocm_solver() {
     for (DevBuf *b : {&rows, &models, &traces, &events, &scan, &brackets, &roots, &unc_q, &tr_q, &counters, &stats,
-                      &overflow,
+                      &overflow, &phase,
                       &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &out_v, &out_flags, &out_levels,
                       &out_counts, &out_group, &out_gflags})
       b->release();
@@ -100,7 +100,8 @@ int chunk_size(const md_rocm_solver *s, long long ntraces) {
 }
 
 // Runs the root pipeline over all traces already prepared on the device.
-int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, double tol, double *v64, float *v32,
+int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, int wave, double tol, double *v64,
+                 float *v32,
                  uint32_t *flags, int32_t *levels, int32_t *counts, double *group, uint32_t *gflags) {
   const int chunk = chunk_size(s, ntraces);
   const size_t qcap = static_cast<size_t>(chunk) * mdr::kEventCap;
@@ -112,6 +113,7 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, doubl
   if (int e = s->tr_q.ensure(qcap * sizeof(uint32_t))) return e;
   if (int e = s->counters.ensure(4 * sizeof(int))) return e;
   if (int e = s->overflow.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
+  if (int e = s->phase.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->stats.ensure(kStatCount * sizeof(unsigned long long))) return e;
   MD_HIP(hipMemsetAsync(s->stats.p, 0, kStatCount * sizeof(unsigned long long), s->stream));
   mdr::PipeArgs a{};
@@ -129,8 +131,11 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int wave, doubl
   a.trough_count = a.counters + 1;
   a.overflow_count = a.counters + 2;
   a.overflow_list = s->overflow.as<int>();
+  a.phase_count = a.counters + 3;
+  a.phase_list = s->phase.as<int>();
   a.queue_cap = static_cast<int>(qcap);
   a.nc = s->cfg.nc;
+  a.max_rows = max_rows;
   a.adaptive = s->cfg.grid;
   a.rho = static_cast<float>(s->cfg.rho);
   a.kmax = kmax;
@@ -300,7 +305,7 @@ int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double 
   MD_HIP(mdr::launch_prep(thickness, vp, vs, rho, bounds, periods, row_periods, B, L, T, s->nmodes, pp,
                           s->rows.as<mdr::PrepRow>(), s->models.as<mdr::ModelInfo>(), s->traces.as<mdr::TraceInfo>(),
                           s->stream));
-  if (int e = run_pipeline(s, ntraces, s->nmodes + 2, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
+  if (int e = run_pipeline(s, ntraces, s->nmodes + 2, s->L, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
                            s->cfg.group_velocity ? group : nullptr, group_flags))
     return e;
   MD_HIP(hipEventRecord(s->ev_end, s->stream));
@@ -372,7 +377,7 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
   }
   if (prediction_flags) std::fill_n(prediction_flags, ntr, 8u);  // bypassed: independent periods
   if (stats && nstats > 0) {
-    unsigned long long st4[kStatCount] = {0, 0, 0, 0};
+    unsigned long long st4[kStatCount] = {0, 0, 0, 0, 0, 0};
     MD_HIP(hipMemcpy(st4, s->stats.p, sizeof(st4), hipMemcpyDeviceToHost));
     for (int i = 0; i < std::min(nstats, kStatCount); ++i) stats[i] = st4[i];
   }
@@ -396,7 +401,7 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
   std::vector<mdr::TraceInfo> traces(static_cast<size_t>(period_offsets[B]));
   const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
   long long out0 = 0;
-  int kmax = 0;
+  int kmax = 0, max_rows = 1;
   for (int m = 0; m < B; ++m) {
     const long long r0 = row_offsets[m], r1 = row_offsets[m + 1];
     const long long p0 = period_offsets[m], p1 = period_offsets[m + 1];
@@ -410,6 +415,7 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
     for (int f = 0; f < T; ++f) traces[p0 + f] = mdr::make_trace(m, f, periods[p0 + f], slow, pp);
     out0 += static_cast<long long>(modes[m]) * T;
     kmax = std::max(kmax, modes[m] + 2);
+    max_rows = std::max(max_rows, L);
   }
   const long long ntr = period_offsets[B];
   const size_t ncell = static_cast<size_t>(out0);
@@ -431,7 +437,7 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
                         st));
   MD_HIP(hipMemcpyAsync(s->traces.p, traces.data(), traces.size() * sizeof(mdr::TraceInfo), hipMemcpyHostToDevice,
                         st));
-  if (int e = run_pipeline(s, ntr, kmax, wave, c_tol_rel, s->out_v.as<double>(), nullptr,
+  if (int e = run_pipeline(s, ntr, kmax, max_rows, wave, c_tol_rel, s->out_v.as<double>(), nullptr,
                            s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(), s->out_counts.as<int32_t>(),
                            want_group ? s->out_group.as<double>() : nullptr,
                            want_group ? s->out_gflags.as<uint32_t>() : nullptr))

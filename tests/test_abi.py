@@ -41,7 +41,9 @@ def test_plugin_attributes_match_reference_loader_contract():
 
 def test_invalid_configuration_is_a_value_error_before_any_device_call():
     with pytest.raises(ValueError, match="nc"):
-        modisp_amd.QeSolver(4, 5, 3, 2, nc=300)
+        modisp_amd.QeSolver(4, 5, 3, 2, nc=2)
+    with pytest.raises(ValueError, match="rho"):
+        modisp_amd.QeSolver(4, 5, 3, 2, rho=.5)
     with pytest.raises(ValueError):
         modisp_amd.QeSolver(4, 300, 3, 2)   # more than 256 rows
     with pytest.raises(ValueError):
@@ -52,7 +54,7 @@ def test_last_error_is_thread_local_text():
     lib = _native.lib()
     cfg = _native.Config()
     lib.md_rocm_default_config(ctypes.byref(cfg))
-    cfg.nc = 3
+    cfg.nc = 2
     assert not lib.md_rocm_create(1, 2, 1, 1, ctypes.byref(cfg))
     assert "nc" in _native.last_error()
 

@@ -212,14 +212,32 @@ def test_halfspace_rayleigh_and_empty_families(md):
 
 # --- full-size, size-independent properties ------------------------------------------------------
 @pytest.mark.parametrize("wave", [0, 1], ids=["rayleigh", "love"])
-def test_full_precision_workload_fp32_equals_fp64_path(md, wave):
+def test_full_precision_workload_fp32_and_fp64_paths(md, wave):
+    """At the full 16384 x 100 size: the FP32-scan path and the plain FP64 path
+    agree on all but a handful of traces, and on every trace where they differ
+    (plus a seeded sample of the others) both equal the oracle."""
     from modisp_amd import workloads
     wl = workloads.precision_batch()
     win = workloads.scan_window(wl.vs)
     a = gpu_solve(md, *(getattr(wl, k) for k in MODEL), wl.periods, win, 5, wave, "float32")
     b = gpu_solve(md, *(getattr(wl, k) for k in MODEL), wl.periods, win, 5, wave, "float64")
-    assert (a["root_count"] >= 0).all()
-    assert_parity(a, b, label=f"fp32 vs fp64 wave {wave}")
+    assert (a["root_count"] >= 0).all() and (b["root_count"] >= 0).all()
+    with np.errstate(invalid="ignore"):
+        rel = np.abs(a["phase_velocity"] - b["phase_velocity"]) / b["phase_velocity"]
+    differ = ((a["root_count"] != b["root_count"]) | np.any(rel > RTOL, axis=1) |
+              np.any(np.isfinite(a["phase_velocity"]) != np.isfinite(b["phase_velocity"]), axis=1))
+    assert differ.sum() <= 1e-4 * differ.size, f"{differ.sum()} traces differ"
+    rng = np.random.default_rng(7 + wave)
+    check = np.argwhere(differ).tolist() + [[int(i), int(f)] for i, f in
+                                            zip(rng.integers(0, wl.B, 64), rng.integers(0, wl.periods.size, 64))]
+    for i, f in check:
+        ref = oracle.solve(*(getattr(wl, k)[i:i + 1] for k in MODEL), wl.periods[f:f + 1], win[i:i + 1], 5, wave,
+                           tol=1e-12, group=2)
+        for path, got in (("fp32", a), ("fp64", b)):
+            one = {k: (v[i:i + 1, ..., f:f + 1] if v is not None and v.ndim == 3 else
+                       v[i:i + 1, f:f + 1] if v is not None and v.ndim == 2 else v)
+                   for k, v in got.items() if k != "timing"}
+            assert_parity(one, ref, label=f"{path} model {i} period {f} wave {wave}")
     v = a["phase_velocity"]
     d = np.diff(v, axis=1)
     assert np.all(d[np.isfinite(d)] > 0)
