@@ -1,14 +1,23 @@
 """Benchmark of the modisp-rocm hot path on one MI355X.
 
-A step is one pass of the multimode root pipeline over the precision
-workload (16384 models of 12 layers + half-space, 100 log-spaced periods
-0.5-60 s, Rayleigh and Love, 5 modes each, analytic FP64 group velocity):
-FP32 uniform 512-sample grid scan with FP64 recheck, FP64 Brent refinement
-to 1e-12 relative, FP64 group velocity. Inputs are resident in HBM before the
-timed region; outputs stay in HBM. `value` is valid roots (finite phase
-cells) per second. The plain FP64 grid path runs the same workload for the
-FP32/FP64 ratio. The CPU baseline is the oracle (a C++17 FP64 restatement of
-the reference CPU backend) on a bounded sample of the same workload.
+Workload: the precision batch -- 16384 seeded monotone models of 12 layers
+plus half-space, 100 log-spaced periods 0.5-60 s, Rayleigh and Love, 5 modes
+each, phase window [0.8 min Vs, Vs_halfspace), FP64 refinement to 1e-12
+relative, analytic FP64 group velocity.
+
+A step is one pass of the multimode root pipeline over the whole batch for
+both wave types (scan -> FP64 recheck / pair probes -> brackets -> FP64
+refinement -> group velocity). Inputs are resident in HBM before the timed
+region and outputs stay there. `value` is valid roots (finite phase cells)
+per second of the FP32-scan path. The same steps are timed for the plain
+FP64 path (identical grid and pipeline, every scan evaluation in FP64) and,
+for reference, for the literal 512-sample uniform grid in both precisions.
+
+`roofline` is for the dominant kernel, the FP32 Rayleigh grid scan: its
+algorithmic FLOP (secular evaluations x layer steps x 160 FLOP per Dunkin
+layer step, counted on the device) divided by its HIP-event time on the
+solver's stream. `cpu_baseline` is the oracle (C++17 FP64 restatement of the
+reference CPU backend) on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -24,9 +33,11 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-METRIC = ("roots/s at ≤1e-9 rel err: FP32-scan+FP64 vs plain FP64; fwd+adjoint models/s")
-FLOP_PER_LAYER = {0: 160.0, 1: 20.0}   # algorithmic FLOP per layer step, Rayleigh / Love
-PEAK = {"f32": 157.3, "f64": 78.6}      # MI355X published vector peaks, TFLOP/s
+METRIC = "roots/s at ≤1e-9 rel err: FP32-scan+FP64 vs plain FP64; fwd+adjoint models/s"
+FLOP_PER_LAYER = {0: 160.0, 1: 20.0}   # algorithmic FLOP per layer step: Dunkin (Rayleigh) / SH (Love)
+PEAK_FP32 = 157.3                      # MI355X published FP32 vector peak, TFLOP/s (packed dual issue)
+PEAK_FP32_UNPACKED = 78.6              # the same without packed FP32 (not used by these kernels)
+TRAFFIC_FILE = ROOT / "profiles" / "scan_kernel_precision_traffic.json"
 
 
 def log(msg):
@@ -36,7 +47,7 @@ def log(msg):
 class DeviceCase:
     """One wave type of the workload resident on the device, with its solver."""
 
-    def __init__(self, modisp_amd, torch, wl, window, wave, scan, group=True):
+    def __init__(self, modisp_amd, torch, wl, window, wave, scan, grid="adaptive", group=True):
         dev = torch.device("cuda", 0)
         self.B, self.L = wl.thickness.shape
         self.T = wl.periods.size
@@ -52,7 +63,8 @@ class DeviceCase:
         self.counts = torch.empty(self.B * self.T, dtype=torch.int32, device=dev)
         self.group = torch.empty(n, dtype=torch.float64, device=dev) if group else None
         self.gflags = torch.empty(n, dtype=torch.int32, device=dev) if group else None
-        self.solver = modisp_amd.QeSolver(self.B, self.L, self.T, self.M, group_velocity=group, scan=scan)
+        self.solver = modisp_amd.QeSolver(self.B, self.L, self.T, self.M, group_velocity=group, scan=scan, grid=grid,
+                                          nc=512)
         self.ptrs = dict(thickness=self.h.data_ptr(), vp=self.vp.data_ptr(), vs=self.vs.data_ptr(),
                          rho=self.rho.data_ptr(), periods=self.periods.data_ptr(), bounds=self.bounds.data_ptr(),
                          velocity=self.velocity.data_ptr(), flags=self.flags.data_ptr(),
@@ -60,48 +72,52 @@ class DeviceCase:
                          group=self.group.data_ptr() if group else None,
                          group_flags=self.gflags.data_ptr() if group else None)
 
-    def run(self):
+    def launch(self):
         self.solver.run_device_batch(self.B, self.ptrs, 1e-12, self.wave)
+
+    def stages(self):
+        """Per-stage device milliseconds of the last run (synchronizes its stream)."""
         self.solver.synchronize()
         return self.solver.last_timing()
 
-    def valid_roots(self, torch):
-        return int(torch.isfinite(self.velocity).sum().item())
-
-    def scan_flop(self):
-        # Every trace evaluates all nc grid nodes through L-1 layer steps.
-        return self.B * self.T * 512 * (self.L - 1) * FLOP_PER_LAYER[self.wave]
+    def valid_roots(self):
+        return int(self.velocity.isfinite().sum().item())
 
     def close(self):
         self.solver.close()
 
 
 def timed_steps(torch, cases, steps, warmup):
+    """W untimed steps, then exactly K timed steps bracketed by device syncs.
+    Returns wall seconds and the mean per-step stage times of every case."""
     for _ in range(warmup):
         for c in cases:
-            c.run()
+            c.launch()
+        for c in cases:
+            c.stages()
     torch.cuda.synchronize()
-    stage = [0.0] * len(cases)
+    stage = [None] * len(cases)
     t0 = time.perf_counter()
     for _ in range(steps):
+        for c in cases:
+            c.launch()
         for i, c in enumerate(cases):
-            ms = c.run()
-            stage[i] += ms[1]  # grid-scan kernel time of this case
+            ms = c.stages()
+            stage[i] = ms if stage[i] is None else stage[i] + ms
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall, [s / steps for s in stage]
 
 
-def cpu_baseline(wl, window, gpu_cases, torch, threads, sample):
-    """The oracle (C++17 FP64 restatement, reference high-accuracy search) on
-    the first `sample` models of the same workload; also checks the GPU roots
-    of those models against it (mode counts, masks, 1e-9 relative)."""
+def cpu_baseline(wl, window, gpu_cases, threads, sample):
+    """The oracle on the first `sample` models of the workload (both waves, all
+    periods, analytic group velocity), timed on the host; also checks the
+    device roots of those models against it."""
     import numpy as np
     import oracle
     oracle.build(force=False)
     sub = wl.subset(np.arange(sample))
-    roots = 0
-    elapsed = 0.0
+    roots, elapsed = 0, 0.0
     agree = {"count_match": 1.0, "mask_match": 1.0, "max_rel_phase": 0.0, "max_rel_group": 0.0}
     for case in gpu_cases:
         t0 = time.perf_counter()
@@ -114,7 +130,8 @@ def cpu_baseline(wl, window, gpu_cases, torch, threads, sample):
         gv = case.velocity.view(B, M, T)[:sample].cpu().numpy()
         gc = case.counts.view(B, T)[:sample].cpu().numpy()
         gg = case.group.view(B, M, T)[:sample].cpu().numpy()
-        agree["count_match"] = min(agree["count_match"], float(np.mean(gc == ref["root_count"])))
+        resolved = ref["root_count"] >= 0
+        agree["count_match"] = min(agree["count_match"], float(np.mean((gc == ref["root_count"])[resolved])))
         agree["mask_match"] = min(agree["mask_match"], float(np.mean(np.isfinite(gv) == np.isfinite(rv))))
         both = np.isfinite(gv) & np.isfinite(rv)
         if both.any():
@@ -125,9 +142,20 @@ def cpu_baseline(wl, window, gpu_cases, torch, threads, sample):
             agree["max_rel_group"] = max(agree["max_rel_group"],
                                          float(np.max(np.abs(gg[gb] - rg[gb]) / np.abs(rg[gb]))))
     return {"value": roots / elapsed, "unit": "roots/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample} models of the workload, Rayleigh+Love, {gpu_cases[0].M} modes, "
-                      f"{gpu_cases[0].T} periods, group velocity; {roots} roots in {elapsed:.2f} s",
+            "sample": f"first {sample} of {wl.B} models of the workload, Rayleigh+Love, {gpu_cases[0].M} modes, "
+                      f"{gpu_cases[0].T} periods, analytic group velocity: {roots} roots in {elapsed:.2f} s "
+                      f"on {threads} threads",
             "gpu_parity_on_sample": agree}
+
+
+def traffic_per_launch():
+    """HBM bytes per scan launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by a copy-kernel calibration), if present."""
+    try:
+        d = json.loads(TRAFFIC_FILE.read_text())
+        return float(d["fetch_bytes_per_dispatch"] + d["write_bytes_per_dispatch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -136,12 +164,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--models", type=int, default=16384)
-    ap.add_argument("--cpu-sample", type=int, default=12)
+    ap.add_argument("--cpu-sample", type=int, default=512, help="models in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-uniform", action="store_true", help="skip the 512-sample uniform-grid comparison")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the workload is "
-              f"embarrassingly parallel over models and is not sharded in this build)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every trace is independent, "
+              f"so the batch would be split by model ranges with no exchange, and this build runs on one GPU",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -155,43 +185,79 @@ def main():
     modisp_amd.set_device(0)
     wl = workloads.precision_batch(B=args.models)
     window = workloads.scan_window(wl.vs)
-    log(f"workload {wl.name}: B={wl.B} L={wl.thickness.shape[1]} T={wl.periods.size} modes={wl.nmodes}")
+    L = wl.thickness.shape[1]
+    log(f"workload {wl.name}: B={wl.B} L={L} T={wl.periods.size} modes={wl.nmodes}")
 
-    fast = [DeviceCase(modisp_amd, torch, wl, window, w, "float32") for w in (0, 1)]
-    wall32, scan32 = timed_steps(torch, fast, args.steps, args.warmup)
-    roots32 = sum(c.valid_roots(torch) for c in fast)
-    log(f"fp32-scan path: {wall32 / args.steps * 1e3:.2f} ms/step, {roots32} roots")
-    plain = [DeviceCase(modisp_amd, torch, wl, window, w, "float64") for w in (0, 1)]
-    wall64, scan64 = timed_steps(torch, plain, args.steps, args.warmup)
-    roots64 = sum(c.valid_roots(torch) for c in plain)
-    log(f"fp64 path: {wall64 / args.steps * 1e3:.2f} ms/step, {roots64} roots")
+    def run_path(scan, grid):
+        cases = [DeviceCase(modisp_amd, torch, wl, window, w, scan, grid) for w in (0, 1)]
+        wall, _ = timed_steps(torch, cases, args.steps, args.warmup)
+        roots = sum(c.valid_roots() for c in cases)
+        # Stage times and work counters from one more pass of each wave type on
+        # its own (in the timed steps the two waves' streams overlap).
+        stages, stats = [], []
+        for c in cases:
+            c.launch()
+            stages.append(c.stages())
+            stats.append(c.solver.last_stats())
+        log(f"{scan}/{grid}: {wall / args.steps * 1e3:.2f} ms/step, {roots} roots; stages R "
+            + " ".join(f"{n}={v:.2f}" for n, v in zip(cases[0].solver.STAGES, stages[0]))
+            + " | L " + " ".join(f"{n}={v:.2f}" for n, v in zip(cases[1].solver.STAGES, stages[1])))
+        return cases, wall, stages, roots, stats
+
+    fast, wall32, st32, roots32, stats32 = run_path("float32", "adaptive")
+    plain, wall64, st64, roots64, stats64 = run_path("float64", "adaptive")
     for c in plain:
         c.close()
-
     value = roots32 * args.steps / wall32
     value64 = roots64 * args.steps / wall64
-    ray = fast[0]
-    flop = ray.scan_flop()
-    achieved = flop / (scan32[0] * 1e-3) / 1e12
-    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK["f32"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK["f32"], "traffic": None,
-                "kernel": "scan_kernel<float,2> (Rayleigh)", "algorithmic_flop_per_launch": flop,
-                "kernel_ms": scan32[0]}
+    uniform = {}
+    if not args.no_uniform:
+        for scan in ("float32", "float64"):
+            cases, wall, stages, roots, _ = run_path(scan, "uniform")
+            uniform[scan] = {"value": roots * args.steps / wall, "unit": "roots/s",
+                             "ms_per_step": wall / args.steps * 1e3}
+            for c in cases:
+                c.close()
+
+    # Roofline of the dominant kernel: the FP32 Rayleigh scan.
+    evals = stats32[0]["scan_nodes"]
+    flop = evals * (L - 1) * FLOP_PER_LAYER[0]
+    scan_ms = float(st32[0][1])
+    achieved = flop / (scan_ms * 1e-3) / 1e12
+    love_flop = stats32[1]["scan_nodes"] * (L - 1) * FLOP_PER_LAYER[1]
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32, "traffic": traffic_per_launch(),
+                "kernel": "scan_kernel<float,64,4> (Rayleigh)",
+                "algorithmic_flop_per_run": flop, "secular_evaluations_per_run": evals,
+                "flop_per_layer_step": FLOP_PER_LAYER[0], "layer_steps_per_evaluation": L - 1,
+                "kernel_ms_per_run": scan_ms, "launches_per_run": 2,
+                "peak_without_packed_fp32": PEAK_FP32_UNPACKED,
+                "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED,
+                "love_scan_tflops": love_flop / (float(st32[1][1]) * 1e-3) / 1e12}
     result = {
         "metric": METRIC, "value": value, "unit": "roots/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall32 / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 scan + f64 refine/group",
-        "data": "synthetic (seeded monotone layered models)",
-        "config": {"workload": "precision_fp32_scan", "models": wl.B, "rows": int(wl.thickness.shape[1]),
-                   "periods": int(wl.periods.size), "waves": "rayleigh+love", "modes": int(wl.nmodes),
-                   "grid": 512, "refine_tol_rel": 1e-12, "group_velocity": "analytic fp64"},
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 scan, f64 recheck/refine/group",
+        "data": "synthetic (seeded monotone layered models, precision workload recipe)",
+        "config": {"workload": "precision: 16384 models x 13 rows, 100 periods 0.5-60 s, Rayleigh+Love, 5 modes, "
+                               "group velocity", "models": wl.B, "rows": int(L), "periods": int(wl.periods.size),
+                   "waves": "rayleigh+love", "modes": int(wl.nmodes), "grid": "adaptive (8 nodes per WKB mode)",
+                   "refine_tol_rel": 1e-12, "group_velocity": "analytic fp64"},
         "fp64_path": {"value": value64, "unit": "roots/s", "ms_per_step": wall64 / args.steps * 1e3,
                       "speedup_fp32_over_fp64": value / value64},
+        "uniform512": {k: v for k, v in uniform.items()},
+        "stages_ms": {"fp32_rayleigh": dict(zip(fast[0].solver.STAGES, map(float, st32[0]))),
+                      "fp32_love": dict(zip(fast[1].solver.STAGES, map(float, st32[1]))),
+                      "fp64_rayleigh": dict(zip(fast[0].solver.STAGES, map(float, st64[0]))),
+                      "fp64_love": dict(zip(fast[1].solver.STAGES, map(float, st64[1])))},
+        "work": {"fp32_rayleigh": stats32[0], "fp32_love": stats32[1]},
         "roofline": roofline,
     }
+    if uniform:
+        result["uniform512"]["speedup_fp32_over_fp64"] = uniform["float32"]["value"] / uniform["float64"]["value"]
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        result["cpu_baseline"] = cpu_baseline(wl, window, fast, torch, threads, args.cpu_sample)
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        result["cpu_baseline"] = cpu_baseline(wl, window, fast, threads, min(args.cpu_sample, wl.B))
     for c in fast:
         c.close()
     print(json.dumps(result))

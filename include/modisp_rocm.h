@@ -109,6 +109,15 @@ int md_rocm_synchronize(md_rocm_solver *s);
  * Returns the number of values written (at most 7). */
 int md_rocm_last_timing(md_rocm_solver *s, double *ms, int n);
 
+/* Work counters of the last run (the stats array md_rocm_run returns, also
+ * for md_rocm_run_device): 0 traces, 1 FP64 rechecks of FP32-uncertain nodes,
+ * 2 hidden-pair probes, 3 FP64 rescans of overflowed FP32 traces, 4 Love phase
+ * searches, 5 secular evaluations of the grid scans. Synchronizes the solver's
+ * stream; returns the number of values written (at most 6) or -1 on error.
+ * No reference counterpart: the reference returns these only from run()
+ * (cpp/cuda/qe.cu:815-821). */
+int md_rocm_last_stats(md_rocm_solver *s, uint64_t *stats, int n);
+
 /* Automatic search window [B, 2] (reference _cpu.bounds, cpp/cpu/solver.cpp:781-790):
  * lower = 1/8 of the slowest Rayleigh/Scholte/Love reference speed unless
  * auto_low == 0 (then `low`), upper = min(high or Vs_halfspace, nextafter(Vs_halfspace, 0)). */

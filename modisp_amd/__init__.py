@@ -250,6 +250,13 @@ class QeSolver:
         _native.lib().md_rocm_last_timing(self.handle, _native.ptr(ms), len(self.STAGES))
         return ms
 
+    def last_stats(self):
+        """Work counters of the last run as a dict keyed by STAT_NAMES."""
+        st = np.zeros(len(STAT_NAMES), dtype=np.uint64)
+        if _native.lib().md_rocm_last_stats(self.handle, _native.ptr(st, _native._u64p), len(STAT_NAMES)) < 0:
+            raise RuntimeError(_native.last_error())
+        return {k: int(v) for k, v in zip(STAT_NAMES, st)}
+
     def close(self):
         if getattr(self, "_handle", None) is not None:
             _native.lib().md_rocm_free(self._handle)

@@ -319,9 +319,11 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
   const float theta = kTroughMax;
   int nb = 0, ne = 0;
   bool overflow = false;
+  int evaluated = 0;
   for (int base = 0;; base += 62) {
     const int j = base + lane;
     const bool active = j < total;
+    evaluated += total - base < 64 ? total - base : 64;
     double c = 0.;
     float f = 0.f, sc = 0.f;
     bool u = false;
@@ -383,14 +385,6 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
       break;
     }
     const bool mine = (em >> lane) & 1;
-    const uint64_t uqm = em & __ballot(kind == kUncertain), tqm = em & __ballot(probe != 0);
-    int qu = 0, qt = 0;
-    if (lane == 0) {
-      if (uqm) qu = atomicAdd(a.unc_count, __popcll(uqm));
-      if (tqm) qt = atomicAdd(a.trough_count, __popcll(tqm));
-    }
-    qu = __shfl(qu, 0, 64);
-    qt = __shfl(qt, 0, 64);
     if (mine) {
       const int slot = ne + __popcll(em & lanes_below(lane));
       Event x;
@@ -411,15 +405,12 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
       x.probe = probe;
       x.pad = 0;
       ev[slot] = x;
-      const uint32_t g = static_cast<uint32_t>(tl * kEventCap + slot);
-      if (kind == kUncertain) a.unc_queue[qu + __popcll(uqm & lanes_below(lane))] = g;
-      if (probe) a.trough_queue[qt + __popcll(tqm & lanes_below(lane))] = g;
     }
     ne += nnew;
     if (done) break;
   }
   if (lane == 0) {
-    *ts = TraceScan{overflow ? 0 : ne, overflow ? kTraceOverflow : 0u, 0, nb, 0, total};
+    *ts = TraceScan{overflow ? 0 : ne, overflow ? kTraceOverflow : 0u, 0, nb, 0, evaluated};
     // An FP32 trace whose events overflow is rescanned on the FP64 grid,
     // which has no uncertain nodes.
     if (overflow && fp32) a.overflow_list[atomicAdd(a.overflow_count, 1)] = tl;
@@ -446,6 +437,50 @@ __global__ __launch_bounds__(64 * W) void rescan_kernel(PipeArgs a) {
   const int n = *a.overflow_count;
   for (int q = blockIdx.x * W + w; q < n; q += gridDim.x * W)
     scan_wave<double, R>(a, __builtin_amdgcn_readfirstlane(a.overflow_list[q]) + a.trace_begin, scratch[w], lane);
+}
+
+// ---------------------------------------------------------------------------
+// Work-queue compaction: one lane per trace counts its uncertain and probed
+// events, a block-wide prefix sum places them, and one atomic per block
+// reserves the block's range in each queue (the scan itself takes no
+// device-wide atomics).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void compact_kernel(PipeArgs a) {
+  __shared__ int part[2][kBlock / 64];
+  __shared__ int base[2];
+  const int tl = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool live = tl < a.trace_end - a.trace_begin;
+  int nu = 0, np = 0, nev = 0;
+  const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
+  if (live) {
+    const TraceScan ts = a.scan[tl];
+    if (ts.status == 0 && !(ts.flags & kTraceOverflow)) nev = ts.nev;
+    for (int e = 0; e < nev; ++e) {
+      nu += ev[e].kind == kUncertain;
+      np += ev[e].probe != 0;
+    }
+  }
+  const int iu = wave_incl_scan(nu, lane), ip = wave_incl_scan(np, lane);
+  if (lane == 63) { part[0][w] = iu; part[1][w] = ip; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int su = 0, sp = 0;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      const int u = part[0][k], p = part[1][k];
+      part[0][k] = su; part[1][k] = sp;
+      su += u; sp += p;
+    }
+    base[0] = su ? atomicAdd(a.unc_count, su) : 0;
+    base[1] = sp ? atomicAdd(a.trough_count, sp) : 0;
+  }
+  __syncthreads();
+  int qu = base[0] + part[0][w] + iu - nu, qp = base[1] + part[1][w] + ip - np;
+  for (int e = 0; e < nev; ++e) {
+    const uint32_t g = static_cast<uint32_t>(tl * kEventCap + e);
+    if (ev[e].kind == kUncertain) a.unc_queue[qu++] = g;
+    if (ev[e].probe != 0) a.trough_queue[qp++] = g;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -488,18 +523,33 @@ __device__ inline NodeValues node_values(const Event *ev, int e, int nev) {
 // and are flagged by the scan itself.)
 __global__ __launch_bounds__(kBlock) void unc_probe_kernel(PipeArgs a) {
   const int n = min(*a.unc_count, a.queue_cap);
-  for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
-    const uint32_t g = a.unc_queue[q];
-    const int tl = g / kEventCap, e = g % kEventCap;
-    const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
-    const Event &x = ev[e];
-    if (x.probe || !(x.c0 == x.c0) || !(x.c2 == x.c2)) continue;
-    const NodeValues v = node_values(ev, e, a.scan[tl].nev);
-    if (signbit(v.l) != signbit(v.c) || signbit(v.r) != signbit(v.c)) continue;
-    if (!(fabs(v.c) < fabs(v.l) && fabs(v.c) < fabs(v.r) && fabs(v.c) < double(kTroughMax))) continue;
-    a.events[g].probe = kTrough;
-    const int slot = atomicAdd(a.trough_count, 1);
-    if (slot < a.queue_cap) a.trough_queue[slot] = g;
+  const int lane = threadIdx.x & 63;
+  // Uniform trip count so that every lane of a wave reaches the ballot.
+  const int stride = gridDim.x * kBlock, first = blockIdx.x * kBlock + (threadIdx.x & ~63);
+  for (int q0 = first; q0 < n; q0 += stride) {
+    const int q = q0 + lane;
+    bool want = false;
+    uint32_t g = 0;
+    if (q < n) {
+      g = a.unc_queue[q];
+      const int tl = g / kEventCap, e = g % kEventCap;
+      const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
+      const Event &x = ev[e];
+      if (!x.probe && x.c0 == x.c0 && x.c2 == x.c2) {
+        const NodeValues v = node_values(ev, e, a.scan[tl].nev);
+        want = signbit(v.l) == signbit(v.c) && signbit(v.r) == signbit(v.c) && fabs(v.c) < fabs(v.l) &&
+               fabs(v.c) < fabs(v.r) && fabs(v.c) < double(kTroughMax);
+      }
+    }
+    const uint64_t m = __ballot(want);
+    if (!m) continue;
+    int slot = 0;
+    if (lane == 0) slot = atomicAdd(a.trough_count, __popcll(m));
+    slot = __shfl(slot, 0, 64) + __popcll(m & lanes_below(lane));
+    if (want) {
+      a.events[g].probe = kTrough;
+      if (slot < a.queue_cap) a.trough_queue[slot] = g;
+    }
   }
 }
 
@@ -993,6 +1043,7 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
       hipLaunchKernelGGL((rescan_kernel<kMaxRows, 1>), dim3(pb), dim3(64), 0, s, pa);
   }
   mark(2);
+  hipLaunchKernelGGL(compact_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   hipLaunchKernelGGL(recheck_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   if (fp32) hipLaunchKernelGGL(unc_probe_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   hipLaunchKernelGGL(trough_kernel, dim3(pb), dim3(kBlock), 0, s, pa);

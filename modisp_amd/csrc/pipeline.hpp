@@ -100,7 +100,7 @@ struct TraceScan {
   int status;   // ModelInfo::empty for empty traces
   int certain;  // certain sign changes seen by the scan
   int count;    // roots found (<= K), -1 unresolved
-  int nodes;    // grid nodes of the trace
+  int nodes;    // secular evaluations of the trace's scan (lane slots, batch overlaps included)
 };
 
 struct Bracket {

@@ -284,6 +284,26 @@ int md_rocm_last_timing(md_rocm_solver *s, double *ms, int n) {
   return k;
 }
 
+int md_rocm_last_stats(md_rocm_solver *s, uint64_t *stats, int n) {
+  if (!s || !stats || n < 1) {
+    fail(2, "null solver or stats");
+    return -1;
+  }
+  const int k = std::min(n, kStatCount);
+  if (!s->stats.p) {
+    for (int i = 0; i < k; ++i) stats[i] = 0;
+    return k;
+  }
+  unsigned long long st[kStatCount];
+  if (hipMemcpyAsync(st, s->stats.p, sizeof(st), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess) {
+    fail(1, "stats copy failed");
+    return -1;
+  }
+  for (int i = 0; i < k; ++i) stats[i] = st[i];
+  return k;
+}
+
 int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
                        const double *rho, int B, const double *periods, int row_periods, const double *bounds,
                        double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,

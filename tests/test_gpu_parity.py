@@ -99,8 +99,46 @@ def test_plain_workload_subset_automatic_window(md):
     assert_parity(got, ref, label="plain automatic")
 
 
+def differing_traces(got, ref):
+    with np.errstate(invalid="ignore"):
+        rel = np.abs(got["phase_velocity"] - ref["phase_velocity"]) / ref["phase_velocity"]
+    return ((got["root_count"] != ref["root_count"]) | np.any(rel > RTOL, axis=1) |
+            np.any(np.isfinite(got["phase_velocity"]) != np.isfinite(ref["phase_velocity"]), axis=1)) & \
+        (ref["root_count"] >= 0)
+
+
+def assert_extra_pairs_are_sign_changes(wl, per, win, got, i, f, modes, wave):
+    """A trace where the device reports roots the oracle's ladder did not find:
+    every extra root is a sign change of the FP64 secular function (the
+    oracle's own evaluator, one part in 1e10 either side), the extras come
+    in close pairs, and with them the device roots are the oracle's roots
+    merged with the extras."""
+    model = [np.ascontiguousarray(getattr(wl, k)[i]) for k in MODEL]
+    deeper = oracle.solve(*(getattr(wl, k)[i:i + 1] for k in MODEL), per[f:f + 1], win[i:i + 1], modes + 6, wave,
+                          tol=1e-12)
+    known = deeper["phase_velocity"][0, :, 0]
+    known = known[np.isfinite(known)]
+    mine = got["phase_velocity"][i, :, f]
+    mine = mine[np.isfinite(mine)]
+    extra = [r for r in mine if not np.any(np.abs(known - r) <= RTOL * r)]
+    assert extra, f"model {i} period {per[f]}: differs without extra roots"
+    for r in extra:
+        fl, fr = oracle.secular(*model, per[f], np.array([r * (1 - 1e-10), r * (1 + 1e-10)]), wave=wave)
+        assert np.signbit(fl) != np.signbit(fr), f"model {i} period {per[f]}: {r} is not a sign change"
+    for a, b in zip(extra[0::2], extra[1::2]):   # each pair sits between two consecutive oracle roots
+        assert not np.any((known > a) & (known < b)), f"model {i} period {per[f]}: extras {a}, {b} not a pair"
+    merged = np.sort(np.concatenate([known, extra]))[:mine.size]
+    np.testing.assert_allclose(mine, merged, rtol=RTOL)
+
+
 def test_sweep_workload_subset(md):
-    """Random-walk models (low-velocity zones, Vp/Vs 1.65-2.0), Rayleigh and Love."""
+    """Random-walk models (low-velocity zones, Vp/Vs 1.65-2.0), Rayleigh and Love.
+
+    Known deviation: the oracle's ladder accepts a trace once two adaptive
+    levels agree, so it can miss a collapsing pair of roots narrower than its
+    finest adaptive spacing; the device's log-scale dip probe reports such a
+    pair. Those traces (at most 1%) are checked by assert_extra_pairs_are_sign_changes;
+    every other trace must match the oracle exactly."""
     from modisp_amd import workloads
     wl = workloads.sweep_batch(B=12, seed=104)
     per = wl.periods[::8]
@@ -109,7 +147,12 @@ def test_sweep_workload_subset(md):
         win = np.stack([np.maximum(b[:, 0], 0.8 * wl.vs.min(axis=1)), b[:, 1]], axis=1)
         got = gpu_solve(md, *(getattr(wl, k) for k in MODEL), per, win, 4, wave, "float32")
         ref = oracle.solve(*(getattr(wl, k) for k in MODEL), per, win, 4, wave, tol=1e-12, group=2)
-        assert_parity(got, ref, label=f"sweep wave {wave}")
+        differ = differing_traces(got, ref)
+        assert differ.sum() <= max(1, 0.01 * differ.size), f"sweep wave {wave}: {differ.sum()} traces differ"
+        for i, f in np.argwhere(differ):
+            assert_extra_pairs_are_sign_changes(wl, per, win, got, i, f, 4, wave)
+        rest = dict(ref, root_count=np.where(differ, -1, ref["root_count"]))
+        assert_parity(got, rest, label=f"sweep wave {wave}")
 
 
 def test_ragged_uneven_batch(md):
