@@ -51,14 +51,14 @@ def assert_parity(got, ref, group=True, label=""):
                                   err_msg=f"{label} root counts")
     gv, rv = got["phase_velocity"], ref["phase_velocity"]
     np.testing.assert_array_equal(np.isfinite(gv)[cells], np.isfinite(rv)[cells], err_msg=f"{label} validity mask")
-    ok = np.isfinite(rv)
+    ok = np.isfinite(rv) & cells
     if ok.any():
         rel = np.abs(gv[ok] - rv[ok]) / rv[ok]
         assert rel.max() <= RTOL, f"{label} phase rel {rel.max():.3e}"
     if group:
         gg, rg = got["group_velocity"], ref["group_velocity"]
         np.testing.assert_array_equal(np.isfinite(gg)[cells], np.isfinite(rg)[cells], err_msg=f"{label} group mask")
-        gok = np.isfinite(rg)
+        gok = np.isfinite(rg) & cells
         if gok.any():
             grel = np.abs(gg[gok] - rg[gok]) / np.abs(rg[gok])
             assert grel.max() <= RTOL, f"{label} group rel {grel.max():.3e}"
