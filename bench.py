@@ -13,6 +13,11 @@ per second of the FP32-scan path. The same steps are timed for the plain
 FP64 path (identical grid and pipeline, every scan evaluation in FP64) and,
 for reference, for the literal 512-sample uniform grid in both precisions.
 
+`fwd_adjoint` is the second half of the metric: models/s of forward root
+solve + misfit loss + adjoint backward on the differentiation workload
+(4096 models x 21 rows, 50 periods, Rayleigh, 3 modes) through the
+torch.autograd binding.
+
 `roofline` is for the dominant kernel, the FP32 Rayleigh grid scan: its
 algorithmic FLOP (secular evaluations x layer steps x 160 FLOP per Dunkin
 layer step, counted on the device) divided by its HIP-event time on the
@@ -148,6 +153,59 @@ def cpu_baseline(wl, window, gpu_cases, threads, sample):
             "gpu_parity_on_sample": agree}
 
 
+def gradient_leg(torch, modisp_amd, workloads, steps, warmup):
+    """Forward + adjoint on the differentiation workload (4096 models of 20
+    layers + half-space, 50 periods 2-40 s, Rayleigh, 3 modes): a step is the
+    device root solve, the misfit loss sum (c - c_target)^2 over valid cells
+    and its backward pass (the adjoint kernel), targets precomputed."""
+    from modisp_amd.autograd import phase_velocity
+    wl = workloads.gradient_batch()
+    dev = torch.device("cuda", 0)
+    B, L = wl.thickness.shape
+    window = workloads.scan_window(wl.vs)
+    solver = modisp_amd.QeSolver(B, L, wl.periods.size, 3)
+    per = torch.tensor(wl.periods, device=dev)
+    win = torch.tensor(window, device=dev)
+    with torch.no_grad():
+        tgt = phase_velocity(*(torch.tensor(x, device=dev) for x in (wl.thickness, wl.vp, wl.extra["target_vs"], wl.rho)),
+                             per, win, nmodes=3)
+    params = [torch.tensor(getattr(wl, k), device=dev, requires_grad=True) for k in ("thickness", "vp", "vs", "rho")]
+
+    def step():
+        for p in params:
+            p.grad = None
+        c = phase_velocity(*params, per, win, nmodes=3, solver=solver)
+        ok = torch.isfinite(c) & torch.isfinite(tgt)
+        loss = ((torch.where(ok, c, 0.) - torch.where(ok, tgt, 0.)) ** 2).sum()
+        loss.backward()
+        return c
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c = step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    roots = int(torch.isfinite(c).sum().item())
+    # adjoint alone: the last backward's share, timed with events on torch's stream
+    c = phase_velocity(*params, per, win, nmodes=3, solver=solver)
+    ok = torch.isfinite(c) & torch.isfinite(tgt)
+    loss = ((torch.where(ok, c, 0.) - torch.where(ok, tgt, 0.)) ** 2).sum()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    loss.backward()
+    e1.record()
+    torch.cuda.synchronize()
+    solver.close()
+    return {"value": B * steps / wall, "unit": "models/s", "ms_per_step": wall / steps * 1e3,
+            "adjoint_ms": e0.elapsed_time(e1), "roots_per_step": roots,
+            "config": {"workload": "differentiation: 4096 models x 21 rows, 50 periods 2-40 s, Rayleigh, 3 modes, "
+                                   "loss sum (c - c_target)^2, targets from Vs + N(0, 0.05)",
+                       "parameters": "h, vp, vs, rho of every row", "dtype": "f32 scan, f64 refine and adjoint"}}
+
+
 def traffic_per_launch():
     """HBM bytes per scan launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (scaled by a copy-kernel calibration), if present."""
@@ -167,6 +225,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=512, help="models in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uniform", action="store_true", help="skip the 512-sample uniform-grid comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every trace is independent, "
@@ -255,6 +314,10 @@ def main():
     }
     if uniform:
         result["uniform512"]["speedup_fp32_over_fp64"] = uniform["float32"]["value"] / uniform["float64"]["value"]
+    if not args.no_adjoint:
+        result["fwd_adjoint"] = gradient_leg(torch, modisp_amd, workloads, args.steps, args.warmup)
+        log(f"fwd+adjoint: {result['fwd_adjoint']['ms_per_step']:.2f} ms/step, "
+            f"{result['fwd_adjoint']['value']:.0f} models/s")
     if not args.no_cpu:
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
         result["cpu_baseline"] = cpu_baseline(wl, window, fast, threads, min(args.cpu_sample, wl.B))

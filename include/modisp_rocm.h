@@ -86,6 +86,27 @@ int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double 
                        double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
                        int32_t *counts, double *group, uint32_t *group_flags);
 
+/* Adjoint of md_rocm_run_device's roots (all pointers DEVICE memory, [B, L]
+ * models, [nf] or [B, nf] periods, [B, nmodes, nf] roots and weights):
+ * grad_p[b, j] = sum over modes and periods of weight * dc/dp_j for
+ * p in (thickness, vp, vs, rho), by the implicit rule dc/dp = -(dF/dp)/(dF/dc)
+ * at each root with the propagator recomputed in FP64 (forward sweep in
+ * wavenumber duals, reverse sweep with the transposed layer matrices).
+ * Roots that are NaN or have zero weight contribute nothing; the water
+ * layer's vs entry gets no gradient. No reference counterpart (the reference
+ * has no gradients); this is the entry point its torch.autograd binding calls. */
+int md_rocm_adjoint(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                    const double *rho, int B, const double *periods, int row_periods, int wave,
+                    const double *roots, const double *weight, double *grad_thickness, double *grad_vp,
+                    double *grad_vs, double *grad_rho);
+
+/* Run this solver's work on `stream` (a hipStream_t, e.g. the caller's
+ * framework stream; NULL is the default stream) instead of its own, or, with
+ * own != 0, return to the solver's own stream. The solver does not own or
+ * destroy a stream set here. No reference counterpart (the reference keeps
+ * one private stream per solver, cpp/cuda/qe.cu:998). */
+int md_rocm_set_stream(md_rocm_solver *s, void *stream, int own);
+
 /* Ragged batch from HOST arrays: model m has rows [row_offsets[m], row_offsets[m+1])
  * of the packed thickness/vp/vs/rho arrays, periods [period_offsets[m],
  * period_offsets[m+1]) and modes[m] reported modes. Outputs are packed per

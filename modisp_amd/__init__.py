@@ -239,6 +239,23 @@ class QeSolver:
             p.get("group"), p.get("group_flags"))
         _native.check(code)
 
+    def adjoint_device_batch(self, B, pointers, wave, row_periods=False):
+        """Weighted root gradients on the device: pointers holds device addresses
+        of thickness, vp, vs, rho, periods, roots, weight (dLoss/dc, layout of
+        roots) and grad_thickness, grad_vp, grad_vs, grad_rho ([B, L] outputs):
+        grad_p = sum over roots of weight * dc/dp (implicit rule, FP64)."""
+        p = pointers
+        _native.check(_native.lib().md_rocm_adjoint(
+            self.handle, p["thickness"], p["vp"], p["vs"], p["rho"], int(B), p["periods"], int(bool(row_periods)),
+            int(wave), p["roots"], p["weight"], p["grad_thickness"], p["grad_vp"], p["grad_vs"], p["grad_rho"]))
+
+    def set_stream(self, stream=None):
+        """Issue this solver's work on a HIP stream given as an integer handle
+        (e.g. torch.cuda.current_stream().cuda_stream, 0 for the default
+        stream); None returns to the solver's own stream."""
+        own = stream is None
+        _native.check(_native.lib().md_rocm_set_stream(self.handle, None if own else int(stream), int(own)))
+
     def synchronize(self):
         _native.check(_native.lib().md_rocm_synchronize(self.handle))
 

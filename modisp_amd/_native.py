@@ -38,7 +38,8 @@ _lib = None
 EXPORTS = ("md_rocm_default_config", "md_rocm_create", "md_rocm_free", "md_rocm_run", "md_rocm_run_device",
            "md_rocm_run_ragged", "md_rocm_synchronize", "md_rocm_last_timing", "md_rocm_bounds",
            "md_rocm_available", "md_rocm_set_device", "md_rocm_memory_info", "md_rocm_device_name",
-           "md_rocm_last_error", "md_rocm_probe_grid", "md_rocm_last_stats")
+           "md_rocm_last_error", "md_rocm_probe_grid", "md_rocm_last_stats", "md_rocm_adjoint",
+           "md_rocm_set_stream")
 
 
 def lib():
@@ -73,6 +74,11 @@ def lib():
     L.md_rocm_synchronize.restype = ctypes.c_int
     L.md_rocm_last_timing.argtypes = [_vp, _dp, ctypes.c_int]
     L.md_rocm_last_timing.restype = ctypes.c_int
+    L.md_rocm_adjoint.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                  _vp, _vp, _vp, _vp]
+    L.md_rocm_adjoint.restype = ctypes.c_int
+    L.md_rocm_set_stream.argtypes = [_vp, _vp, ctypes.c_int]
+    L.md_rocm_set_stream.restype = ctypes.c_int
     L.md_rocm_last_stats.argtypes = [_vp, _u64p, ctypes.c_int]
     L.md_rocm_last_stats.restype = ctypes.c_int
     L.md_rocm_bounds.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -8,7 +8,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
 LIBRARY = HERE / "libmodisp_rocm.so"
-SOURCES = [HERE / "csrc" / "kernels.hip", HERE / "csrc" / "solver.cpp", HERE / "csrc" / "bounds.cpp"]
+SOURCES = [HERE / "csrc" / "kernels.hip", HERE / "csrc" / "adjoint.hip", HERE / "csrc" / "solver.cpp",
+           HERE / "csrc" / "bounds.cpp"]
 HEADERS = [HERE / "csrc" / "secular.hpp", HERE / "csrc" / "pipeline.hpp", ROOT / "include" / "modisp_rocm.h"]
 
 
@@ -23,7 +24,7 @@ def command(output: Path = LIBRARY, arch: str = "gfx950") -> list[str]:
             # Packed FP32 instructions are disabled for every HIP translation unit.
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", str(ROOT / "include"), "-o", str(output),
-            str(SOURCES[0]), "-x", "hip", *[str(s) for s in SOURCES[1:]]]
+            str(SOURCES[0]), str(SOURCES[1]), "-x", "hip", *[str(s) for s in SOURCES[2:]]]
 
 
 def build(force: bool = True, quiet: bool = True) -> Path:

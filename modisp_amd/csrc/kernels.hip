@@ -975,6 +975,10 @@ __global__ void reset_kernel(int *counters, int n) {
   if (threadIdx.x < n) counters[threadIdx.x] = 0;
 }
 
+__global__ void reset_stats_kernel(unsigned long long *stats, int n) {
+  if (threadIdx.x < n) stats[threadIdx.x] = 0ull;
+}
+
 // Accumulate this chunk's work counters into the run statistics:
 // [0] traces, [1] FP64 rechecks, [2] pair probes, [3] FP64 rescans,
 // [4] Love phase searches, [5] evaluated scan nodes.
@@ -1031,6 +1035,10 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
     if (marks) (void)hipEventRecord(marks[i], s);
   };
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, sa.counters, 4);
+  // Run statistics accumulate over chunks; cleared by a kernel (not a memset
+  // node) so that the sequence stays capturable.
+  if (pa.stats && pa.trace_begin == 0)
+    hipLaunchKernelGGL(reset_stats_kernel, dim3(1), dim3(64), 0, s, pa.stats, 6);
   mark(0);
   hipError_t err = fp32 ? launch_scan_t<float>(sa, s) : launch_scan_t<double>(sa, s);
   if (err != hipSuccess) return err;

@@ -240,6 +240,17 @@ hipError_t launch_prep(const double *h, const double *vp, const double *vs, cons
 }  // namespace mdr
 
 namespace mdr {
+// Adjoint launch (adjoint.hip): one 64-lane block per model at a time.
+struct AdjointArgs {
+  const double *h, *vp, *vs, *rho;  // [B, L] models
+  const double *periods;            // [T] or [B, T] (row_periods)
+  const double *roots;              // [B, modes, T] phase velocities (NaN: no root)
+  const double *weight;             // [B, modes, T] dLoss/dc
+  double *gh, *gvp, *gvs, *grho;    // [B, L] sum over roots of weight * dc/dp
+  double *tape;                     // [L * 6 * tape_lanes], lane-interleaved
+  int B, L, T, modes, row_periods, wave, tape_lanes;
+};
+hipError_t launch_adjoint(const AdjointArgs &a, int blocks, hipStream_t s);
 hipError_t launch_probe(const TraceInfo *traces, const ModelInfo *models, const PrepRow *rows, int ntraces, int nc,
                         int wave, int fp32, double *out, hipStream_t s);
 }  // namespace mdr

@@ -195,9 +195,16 @@ MDR_HD Wave<T> wave_factor(T k, T kx, T h) {
   return o;
 }
 
-// One Rayleigh layer: e <- e * C(layer), then exact rescale.
+// The 15 distinct entries of Dunkin's compound matrix of one layer (scaled by
+// the layer's attenuation e^-(p+q)), rows/columns laid out as
+// [A E tJ F O; D B tH N F; I G K H J; C M tG B E; Lq C tI D A], t = -2 k^2.
 template <class T>
-MDR_HD T rayleigh_layer(const Layer<T> &ly, T k, T k2, T e[5]) {
+struct Compound {
+  T A, B, C, D, E, F, G, H, I, J, K, Lq, M, N, O, t;
+};
+
+template <class T>
+MDR_HD Compound<T> rayleigh_compound(const Layer<T> &ly, T k, T k2) {
   const Wave<T> P = wave_factor(k, ly.ka, ly.h);
   const Wave<T> S = wave_factor(k, ly.kb, ly.h);
   const T a0 = P.att * S.att;
@@ -207,30 +214,57 @@ MDR_HD T rayleigh_layer(const Layer<T> &ly, T k, T k2, T e[5]) {
   const T gmgmk = gam * gk, gmgm1 = gam * gamm1, gm1sq = gamm1 * gamm1;
   const T rho = ly.rho, irho = ly.irho, irho2 = ly.irho2;
   const T a0pq = a0 - cpcq, k4 = k2 * k2;
-  // The 15 distinct entries of Dunkin's compound matrix, rows/columns laid out as
-  // [A D I C L; E B G M C; tJ tH K tG tI; F N H B D; O F J E A], t = -2 k^2.
-  const T A = cpcq - T(2) * gmgm1 * a0pq - gmgmk * xz - k2 * gm1sq * wy;
-  const T D = (k2 * cpy - cqx) * irho;
-  const T I = -(twgm1 * a0pq + gk * xz + k2 * gamm1 * wy) * irho;
-  const T C = (cpz - k2 * cqw) * irho;
-  const T Lq = -(T(2) * k2 * a0pq + xz + k4 * wy) * irho2;
-  const T E = (gmgmk * cpz - gm1sq * cqw) * rho;
-  const T B = cpcq;
-  const T G = gk * cpz - gamm1 * cqw;
-  const T M = -wz;
-  const T N = -xy;
-  const T F = (gm1sq * cpy - gmgmk * cqx) * rho;
-  const T H = gamm1 * cpy - gk * cqx;
-  const T O = -(T(2) * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * (rho * rho);
-  const T J = -(gk * gamm1 * twgm1 * a0pq + gam * gk * gk * xz + gamm1 * gm1sq * wy) * rho;
-  const T K = a0 + T(2) * (cpcq - A);
-  const T e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3], e4 = e[4];
-  const T e2t = e2 * (T(-2) * k2);
-  e[0] = m_fma(A, e0, m_fma(E, e1, m_fma(F, e3, m_fma(O, e4, J * e2t))));
-  e[1] = m_fma(D, e0, m_fma(B, e1, m_fma(N, e3, m_fma(F, e4, H * e2t))));
-  e[2] = m_fma(I, e0, m_fma(G, e1, m_fma(K, e2, m_fma(H, e3, J * e4))));
-  e[3] = m_fma(C, e0, m_fma(M, e1, m_fma(B, e3, m_fma(E, e4, G * e2t))));
-  e[4] = m_fma(Lq, e0, m_fma(C, e1, m_fma(D, e3, m_fma(A, e4, I * e2t))));
+  Compound<T> q;
+  q.A = cpcq - T(2) * gmgm1 * a0pq - gmgmk * xz - k2 * gm1sq * wy;
+  q.D = (k2 * cpy - cqx) * irho;
+  q.I = -(twgm1 * a0pq + gk * xz + k2 * gamm1 * wy) * irho;
+  q.C = (cpz - k2 * cqw) * irho;
+  q.Lq = -(T(2) * k2 * a0pq + xz + k4 * wy) * irho2;
+  q.E = (gmgmk * cpz - gm1sq * cqw) * rho;
+  q.B = cpcq;
+  q.G = gk * cpz - gamm1 * cqw;
+  q.M = -wz;
+  q.N = -xy;
+  q.F = (gm1sq * cpy - gmgmk * cqx) * rho;
+  q.H = gamm1 * cpy - gk * cqx;
+  q.O = -(T(2) * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * (rho * rho);
+  q.J = -(gk * gamm1 * twgm1 * a0pq + gam * gk * gk * xz + gamm1 * gm1sq * wy) * rho;
+  q.K = a0 + T(2) * (cpcq - q.A);
+  q.t = T(-2) * k2;
+  return q;
+}
+
+// n = C e (the propagation step, FMA contraction of the 15 entries).
+template <class T, class U>
+MDR_HD void compound_apply(const Compound<T> &q, const U e[5], T n[5]) {
+  const U e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3], e4 = e[4];
+  const T e2t = e2 * q.t;
+  n[0] = m_fma(q.A, T(e0), m_fma(q.E, T(e1), m_fma(q.F, T(e3), m_fma(q.O, T(e4), q.J * e2t))));
+  n[1] = m_fma(q.D, T(e0), m_fma(q.B, T(e1), m_fma(q.N, T(e3), m_fma(q.F, T(e4), q.H * e2t))));
+  n[2] = m_fma(q.I, T(e0), m_fma(q.G, T(e1), m_fma(q.K, T(e2), m_fma(q.H, T(e3), q.J * T(e4)))));
+  n[3] = m_fma(q.C, T(e0), m_fma(q.M, T(e1), m_fma(q.B, T(e3), m_fma(q.E, T(e4), q.G * e2t))));
+  n[4] = m_fma(q.Lq, T(e0), m_fma(q.C, T(e1), m_fma(q.D, T(e3), m_fma(q.A, T(e4), q.I * e2t))));
+}
+
+// l <- l C (row vector times the compound matrix: the adjoint step).
+template <class T>
+MDR_HD void compound_apply_transpose(const Compound<T> &q, T l[5]) {
+  const T l0 = l[0], l1 = l[1], l2 = l[2], l3 = l[3], l4 = l[4];
+  l[0] = q.A * l0 + q.D * l1 + q.I * l2 + q.C * l3 + q.Lq * l4;
+  l[1] = q.E * l0 + q.B * l1 + q.G * l2 + q.M * l3 + q.C * l4;
+  l[2] = q.t * (q.J * l0 + q.H * l1 + q.G * l3 + q.I * l4) + q.K * l2;
+  l[3] = q.F * l0 + q.N * l1 + q.H * l2 + q.B * l3 + q.D * l4;
+  l[4] = q.O * l0 + q.F * l1 + q.J * l2 + q.E * l3 + q.A * l4;
+}
+
+// One Rayleigh layer: e <- C(layer) e, then rescale; returns the scale.
+template <class T>
+MDR_HD T rayleigh_layer(const Layer<T> &ly, T k, T k2, T e[5]) {
+  const Compound<T> q = rayleigh_compound(ly, k, k2);
+  T n[5];
+  compound_apply(q, e, n);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) e[i] = n[i];
   return rescale<5>(e);
 }
 
@@ -289,6 +323,28 @@ MDR_HD T love_layer(const Layer<T> &ly, T k, T e[2]) {
   e[0] = u;
   e[1] = v;
   return rescale<2>(e);
+}
+
+// The SH layer matrix [[c, mu x], [w / mu, c]] and its two products.
+template <class T>
+struct ShMatrix {
+  T c, mx, wi;
+};
+template <class T>
+MDR_HD ShMatrix<T> love_matrix(const Layer<T> &ly, T k) {
+  const Wave<T> S = wave_factor(k, ly.kb, ly.h);
+  return ShMatrix<T>{S.c, ly.rho * S.x, S.w * ly.irho};
+}
+template <class T, class U>
+MDR_HD void sh_apply(const ShMatrix<T> &q, const U e[2], T n[2]) {
+  n[0] = q.c * T(e[0]) + q.mx * T(e[1]);
+  n[1] = q.wi * T(e[0]) + q.c * T(e[1]);
+}
+template <class T>
+MDR_HD void sh_apply_transpose(const ShMatrix<T> &q, T l[2]) {
+  const T l0 = l[0], l1 = l[1];
+  l[0] = q.c * l0 + q.wi * l1;
+  l[1] = q.mx * l0 + q.c * l1;
 }
 
 // Love F: half-space state (mu r, 1) -- the reference's (rho r, 1/vs^2) times

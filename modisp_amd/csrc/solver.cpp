@@ -68,9 +68,10 @@ struct md_rocm_solver {
   int capacity = 0, L = 0, nf = 0, nmodes = 0;
   int device = 0;
   int cus = 256;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream work is issued on
+  hipStream_t own_stream = nullptr;  // the solver's own stream
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats, overflow, phase;
+  DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats, overflow, phase, tape;
   DevBuf in_h, in_vp, in_vs, in_rho, in_per, in_bounds;
   DevBuf out_v, out_flags, out_levels, out_counts, out_group, out_gflags;
   double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -79,14 +80,14 @@ struct md_rocm_solver {
 
   ~md_rocm_solver() {
     for (DevBuf *b : {&rows, &models, &traces, &events, &scan, &brackets, &roots, &unc_q, &tr_q, &counters, &stats,
-                      &overflow, &phase,
+                      &overflow, &phase, &tape,
                       &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &out_v, &out_flags, &out_levels,
                       &out_counts, &out_group, &out_gflags})
       b->release();
     for (hipEvent_t e : marks) (void)hipEventDestroy(e);
     if (ev_begin) (void)hipEventDestroy(ev_begin);
     if (ev_end) (void)hipEventDestroy(ev_end);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 };
 
@@ -115,7 +116,6 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   if (int e = s->overflow.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->phase.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->stats.ensure(kStatCount * sizeof(unsigned long long))) return e;
-  MD_HIP(hipMemsetAsync(s->stats.p, 0, kStatCount * sizeof(unsigned long long), s->stream));
   mdr::PipeArgs a{};
   a.traces = s->traces.as<mdr::TraceInfo>();
   a.models = s->models.as<mdr::ModelInfo>();
@@ -252,12 +252,13 @@ md_rocm_solver *md_rocm_create(int capacity, int L, int nf, int nmodes, const md
   }
   hipDeviceProp_t p;
   if (hipGetDeviceProperties(&p, s->device) == hipSuccess) s->cus = p.multiProcessorCount;
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&s->ev_begin) != hipSuccess || hipEventCreate(&s->ev_end) != hipSuccess) {
     delete s;
     fail(1, "stream/event creation failed");
     return nullptr;
   }
+  s->stream = s->own_stream;
   return s;
 }
 
@@ -302,6 +303,37 @@ int md_rocm_last_stats(md_rocm_solver *s, uint64_t *stats, int n) {
   }
   for (int i = 0; i < k; ++i) stats[i] = st[i];
   return k;
+}
+
+int md_rocm_set_stream(md_rocm_solver *s, void *stream, int own) {
+  if (!s) return fail(2, "null solver");
+  s->stream = own ? s->own_stream : static_cast<hipStream_t>(stream);
+  return 0;
+}
+
+int md_rocm_adjoint(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                    const double *rho, int B, const double *periods, int row_periods, int wave,
+                    const double *roots, const double *weight, double *grad_thickness, double *grad_vp,
+                    double *grad_vs, double *grad_rho) {
+  if (!s) return fail(2, "null solver");
+  if (B < 1 || B > s->capacity) return fail(2, "B must be 1..capacity");
+  if (wave != 0 && wave != 1) return fail(2, "wave must be 0 (Rayleigh) or 1 (Love)");
+  if (!thickness || !vp || !vs || !rho || !periods || !roots || !weight || !grad_thickness || !grad_vp ||
+      !grad_vs || !grad_rho)
+    return fail(2, "null device pointer");
+  const int blocks = std::max(1, std::min(B, s->cus * 8));
+  const int lanes = blocks * 64;
+  if (int e = s->tape.ensure(static_cast<size_t>(lanes) * s->L * 6 * sizeof(double))) return e;
+  mdr::AdjointArgs a{};
+  a.h = thickness; a.vp = vp; a.vs = vs; a.rho = rho;
+  a.periods = periods; a.roots = roots; a.weight = weight;
+  a.gh = grad_thickness; a.gvp = grad_vp; a.gvs = grad_vs; a.grho = grad_rho;
+  a.tape = s->tape.as<double>();
+  a.B = B; a.L = s->L; a.T = s->nf; a.modes = s->nmodes; a.row_periods = row_periods ? 1 : 0; a.wave = wave;
+  a.tape_lanes = lanes;
+  const hipError_t err = mdr::launch_adjoint(a, blocks, s->stream);
+  if (err != hipSuccess) return fail(1, std::string("adjoint launch: ") + hipGetErrorString(err));
+  return 0;
 }
 
 int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
