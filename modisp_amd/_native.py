@@ -47,6 +47,13 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    # PyTorch ships its own HIP runtime under the same soname. Loading torch
+    # first makes the process use that one runtime for both; loading this
+    # library first would leave torch unable to initialise its devices.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not LIBRARY.exists():
         raise BackendUnavailableError(
             f"{LIBRARY} is not built; run `python -m modisp_amd.build` (hipcc, gfx950)")

@@ -375,6 +375,9 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
   if (!thickness || !vp || !vs || !rho || !periods || !bounds || !velocity || !flags || !levels || !counts)
     return fail(2, "null input or output pointer");
   if (!(f_accept >= 0.)) return fail(2, "f_accept must be non-negative");
+  // |F| <= 1 always holds for this backend's normalised secular function; a
+  // tighter acceptance threshold would need the reference's normalisation.
+  if (f_accept < 1.) return fail(2, "f_accept < 1 is not supported by the ROCm backend");
   const int L = s->L, T = s->nf, M = s->nmodes;
   const size_t nrow = static_cast<size_t>(B) * L, ncell = static_cast<size_t>(B) * M * T,
                ntr = static_cast<size_t>(B) * T;
