@@ -46,6 +46,11 @@ typedef struct md_rocm_config {
   int chunk_traces;        /* traces per device chunk (0 = automatic) */
   int grid;                /* 1: density-adaptive nodes (default), 0: uniform nc-node grid */
   double rho;              /* adaptive grid: nodes per WKB mode (default 8) */
+  int graph;               /* 1: md_rocm_run_device captures its launch sequence in a hipGraph
+                              and replays it for repeated calls with identical arguments */
+  int scan_waves;          /* traces (wavefronts) per scan block: 1, 2, 4 or 8 (FP64: <= 4);
+                              0 = 4, or what md_rocm_autotune picked */
+  int consumer_blocks;     /* persistent consumer blocks per compute unit, 1..64 (0 = 8) */
 } md_rocm_config;
 
 /* Defaults for every field. */
@@ -99,6 +104,18 @@ int md_rocm_adjoint(md_rocm_solver *s, const double *thickness, const double *vp
                     const double *rho, int B, const double *periods, int row_periods, int wave,
                     const double *roots, const double *weight, double *grad_thickness, double *grad_vp,
                     double *grad_vs, double *grad_rho);
+
+/* Tile autotuner: runs md_rocm_run_device's pipeline on the given DEVICE
+ * batch (arguments as md_rocm_run_device; outputs are overwritten) for each
+ * candidate (traces per scan block x persistent consumer blocks per CU),
+ * keeps the fastest in the solver's configuration and returns it through
+ * scan_waves / consumer_blocks (either may be NULL). No reference
+ * counterpart (the reference fixes its launch shapes, cpp/cuda/qe.cu:962-964). */
+int md_rocm_autotune(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                     const double *rho, int B, const double *periods, int row_periods, const double *bounds,
+                     double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
+                     int32_t *counts, double *group, uint32_t *group_flags, int *scan_waves,
+                     int *consumer_blocks);
 
 /* Run this solver's work on `stream` (a hipStream_t, e.g. the caller's
  * framework stream; NULL is the default stream) instead of its own, or, with

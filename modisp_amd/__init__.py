@@ -120,7 +120,8 @@ class QeSolver:
     def __init__(self, B, L, nf, nmodes, accuracy=0, period_threshold=0., close_root_rtol=.005, shape=False,
                  sampling_rtol=.001, secular=0, refinement=1, love_phase=1, single_precision=False,
                  group_velocity=False, group_rtol=.001, continuation=False, anchor_stride=4, *, scan="float32",
-                 grid="adaptive", rho=8., nc=512, tau_floor=None, tau_phase=None, chunk_traces=0):
+                 grid="adaptive", rho=8., nc=512, tau_floor=None, tau_phase=None, chunk_traces=0, graph=False,
+                 scan_waves=0, consumer_blocks=0):
         if grid not in ("adaptive", "uniform"):
             raise ValueError("grid must be adaptive or uniform")
         if scan not in ("float32", "float64"):
@@ -145,6 +146,9 @@ class QeSolver:
         cfg.chunk_traces = int(chunk_traces)
         cfg.grid = int(grid == "adaptive")
         cfg.rho = float(rho)
+        cfg.graph = int(bool(graph))
+        cfg.scan_waves = int(scan_waves)
+        cfg.consumer_blocks = int(consumer_blocks)
         self.capacity, self.L, self.nf, self.nmodes = int(B), int(L), int(nf), int(nmodes)
         self.single_precision = bool(single_precision)
         self.scan = scan
@@ -238,6 +242,18 @@ class QeSolver:
             p["bounds"], float(c_tol_rel), int(wave), p["velocity"], p["flags"], p["levels"], p["counts"],
             p.get("group"), p.get("group_flags"))
         _native.check(code)
+
+    def autotune_device_batch(self, B, pointers, c_tol_rel, wave, row_periods=False):
+        """Time the launch-shape candidates on this device batch (arguments as
+        run_device_batch; outputs are overwritten) and keep the fastest.
+        Returns (traces per scan block, consumer blocks per CU)."""
+        p = pointers
+        w, b = ctypes.c_int(0), ctypes.c_int(0)
+        _native.check(_native.lib().md_rocm_autotune(
+            self.handle, p["thickness"], p["vp"], p["vs"], p["rho"], int(B), p["periods"], int(bool(row_periods)),
+            p["bounds"], float(c_tol_rel), int(wave), p["velocity"], p["flags"], p["levels"], p["counts"],
+            p.get("group"), p.get("group_flags"), ctypes.byref(w), ctypes.byref(b)))
+        return w.value, b.value
 
     def adjoint_device_batch(self, B, pointers, wave, row_periods=False):
         """Weighted root gradients on the device: pointers holds device addresses

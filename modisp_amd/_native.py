@@ -23,7 +23,8 @@ class Config(ctypes.Structure):
                 ("single_precision", ctypes.c_int), ("group_velocity", ctypes.c_int),
                 ("group_rtol", ctypes.c_double), ("continuation", ctypes.c_int), ("anchor_stride", ctypes.c_int),
                 ("nc", ctypes.c_int), ("tau_floor", ctypes.c_double), ("tau_phase", ctypes.c_double),
-                ("chunk_traces", ctypes.c_int), ("grid", ctypes.c_int), ("rho", ctypes.c_double)]
+                ("chunk_traces", ctypes.c_int), ("grid", ctypes.c_int), ("rho", ctypes.c_double),
+                ("graph", ctypes.c_int), ("scan_waves", ctypes.c_int), ("consumer_blocks", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -39,7 +40,7 @@ EXPORTS = ("md_rocm_default_config", "md_rocm_create", "md_rocm_free", "md_rocm_
            "md_rocm_run_ragged", "md_rocm_synchronize", "md_rocm_last_timing", "md_rocm_bounds",
            "md_rocm_available", "md_rocm_set_device", "md_rocm_memory_info", "md_rocm_device_name",
            "md_rocm_last_error", "md_rocm_probe_grid", "md_rocm_last_stats", "md_rocm_adjoint",
-           "md_rocm_set_stream")
+           "md_rocm_set_stream", "md_rocm_autotune")
 
 
 def lib():
@@ -85,6 +86,10 @@ def lib():
                                   _vp, _vp, _vp, _vp]
     L.md_rocm_adjoint.restype = ctypes.c_int
     L.md_rocm_set_stream.argtypes = [_vp, _vp, ctypes.c_int]
+    L.md_rocm_autotune.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_double,
+                                   ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]
+    L.md_rocm_autotune.restype = ctypes.c_int
     L.md_rocm_set_stream.restype = ctypes.c_int
     L.md_rocm_last_stats.argtypes = [_vp, _u64p, ctypes.c_int]
     L.md_rocm_last_stats.restype = ctypes.c_int

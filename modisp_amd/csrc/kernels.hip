@@ -1016,14 +1016,27 @@ namespace mdr {
 
 // Models of up to kAdaptRows rows: four traces per block; deeper models (up to
 // kMaxRows) take the large-scratch variant, one trace per block.
+template <class T, int W>
+static void launch_scan_w(const PipeArgs &a, int n, hipStream_t s) {
+  hipLaunchKernelGGL((scan_kernel<T, kAdaptRows, W>), dim3((n + W - 1) / W), dim3(64 * W), 0, s, a);
+}
+
 template <class T>
 static hipError_t launch_scan_t(const PipeArgs &a, hipStream_t s) {
   const int n = a.trace_end - a.trace_begin;
   if (n <= 0) return hipSuccess;
-  if (a.max_rows <= kAdaptRows)
-    hipLaunchKernelGGL((scan_kernel<T, kAdaptRows, kWaves>), dim3((n + kWaves - 1) / kWaves), dim3(kBlock), 0, s, a);
-  else
+  if (a.max_rows > kAdaptRows) {
     hipLaunchKernelGGL((scan_kernel<T, kMaxRows, 1>), dim3(n), dim3(64), 0, s, a);
+  } else if (a.scan_waves == 1) {
+    launch_scan_w<T, 1>(a, n, s);
+  } else if (a.scan_waves == 2) {
+    launch_scan_w<T, 2>(a, n, s);
+  } else if (a.scan_waves == 8) {
+    if constexpr (sizeof(T) == 4) launch_scan_w<T, 8>(a, n, s);  // 8 FP64 scratches exceed 64 KB of LDS
+    else launch_scan_w<T, 4>(a, n, s);
+  } else {
+    launch_scan_w<T, 4>(a, n, s);
+  }
   return hipGetLastError();
 }
 

@@ -128,6 +128,7 @@ struct PipeArgs {
   int trace_begin, trace_end;
   int nc;
   int max_rows;   // largest row count of any model in the batch (<= kMaxRows)
+  int scan_waves; // traces per scan block (1, 2, 4, 8)
   int adaptive;   // 1: density-adaptive grid, 0: uniform nc-node grid
   float rho;      // adaptive grid: nodes per WKB mode
   int kmax;

@@ -77,6 +77,10 @@ struct md_rocm_solver {
   double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<hipEvent_t> marks;  // kStageMarks per chunk of the last run
   int marked_chunks = 0;
+  bool capturing = false;          // recording the device-path sequence into a graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uintptr_t> graph_key;  // arguments the captured graph was built for
 
   ~md_rocm_solver() {
     for (DevBuf *b : {&rows, &models, &traces, &events, &scan, &brackets, &roots, &unc_q, &tr_q, &counters, &stats,
@@ -84,6 +88,8 @@ struct md_rocm_solver {
                       &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &out_v, &out_flags, &out_levels,
                       &out_counts, &out_group, &out_gflags})
       b->release();
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     for (hipEvent_t e : marks) (void)hipEventDestroy(e);
     if (ev_begin) (void)hipEventDestroy(ev_begin);
     if (ev_end) (void)hipEventDestroy(ev_end);
@@ -136,6 +142,7 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   a.queue_cap = static_cast<int>(qcap);
   a.nc = s->cfg.nc;
   a.max_rows = max_rows;
+  a.scan_waves = s->cfg.scan_waves > 0 ? s->cfg.scan_waves : 4;
   a.adaptive = s->cfg.grid;
   a.rho = static_cast<float>(s->cfg.rho);
   a.kmax = kmax;
@@ -151,20 +158,20 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   a.group = group;
   a.group_flags = gflags;
   a.stats = s->stats.as<unsigned long long>();
-  const int persistent = s->cus * 8;
+  const int persistent = s->cus * (s->cfg.consumer_blocks > 0 ? s->cfg.consumer_blocks : 8);
   const int nchunks = static_cast<int>((ntraces + chunk - 1) / chunk);
   while (static_cast<int>(s->marks.size()) < nchunks * mdr::kStageMarks) {
     hipEvent_t e;
     MD_HIP(hipEventCreate(&e));
     s->marks.push_back(e);
   }
-  s->marked_chunks = nchunks;
+  s->marked_chunks = s->capturing ? 0 : nchunks;
   int c = 0;
   for (long long begin = 0; begin < ntraces; begin += chunk, ++c) {
     a.trace_begin = static_cast<int>(begin);
     a.trace_end = static_cast<int>(std::min<long long>(ntraces, begin + chunk));
     const hipError_t err = mdr::launch_pipeline(a, a, a.fp32 != 0, group != nullptr, persistent, s->stream,
-                                                s->marks.data() + c * mdr::kStageMarks);
+                                                s->capturing ? nullptr : s->marks.data() + c * mdr::kStageMarks);
     if (err != hipSuccess) return fail(1, std::string("kernel launch: ") + hipGetErrorString(err));
   }
   return 0;
@@ -178,6 +185,10 @@ int check_config(const md_rocm_config &c) {
   if (!(c.group_rtol > 0. && c.group_rtol < 1.)) return fail(2, "group_rtol must be in (0, 1)");
   if (c.continuation && c.anchor_stride < 2) return fail(2, "anchor_stride must be at least 2");
   if (!(c.tau_floor >= 0.) || !(c.tau_phase >= 0.)) return fail(2, "FP32 band parameters must be non-negative");
+  if (c.graph != 0 && c.graph != 1) return fail(2, "graph must be 0 or 1");
+  if (c.scan_waves != 0 && c.scan_waves != 1 && c.scan_waves != 2 && c.scan_waves != 4 && c.scan_waves != 8)
+    return fail(2, "scan_waves must be 0, 1, 2, 4 or 8");
+  if (c.consumer_blocks < 0 || c.consumer_blocks > 64) return fail(2, "consumer_blocks must be 0..64");
   return 0;
 }
 
@@ -336,6 +347,23 @@ int md_rocm_adjoint(md_rocm_solver *s, const double *thickness, const double *vp
   return 0;
 }
 
+namespace {
+// Prep + pipeline of a device batch on s->stream (eager, or being captured).
+int device_sequence(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                    const double *rho, int B, const double *periods, int row_periods, const double *bounds,
+                    double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
+                    int32_t *counts, double *group, uint32_t *group_flags) {
+  const int L = s->L, T = s->nf;
+  const long long ntraces = static_cast<long long>(B) * T;
+  const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
+  MD_HIP(mdr::launch_prep(thickness, vp, vs, rho, bounds, periods, row_periods, B, L, T, s->nmodes, pp,
+                          s->rows.as<mdr::PrepRow>(), s->models.as<mdr::ModelInfo>(), s->traces.as<mdr::TraceInfo>(),
+                          s->stream));
+  return run_pipeline(s, ntraces, s->nmodes + 2, s->L, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
+                      s->cfg.group_velocity ? group : nullptr, group_flags);
+}
+}  // namespace
+
 int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
                        const double *rho, int B, const double *periods, int row_periods, const double *bounds,
                        double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
@@ -352,15 +380,92 @@ int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double 
   if (int e = s->rows.ensure(static_cast<size_t>(B) * L * sizeof(mdr::PrepRow))) return e;
   if (int e = s->models.ensure(static_cast<size_t>(B) * sizeof(mdr::ModelInfo))) return e;
   if (int e = s->traces.ensure(static_cast<size_t>(ntraces) * sizeof(mdr::TraceInfo))) return e;
+  (void)ntraces;
+  double tolbits = c_tol_rel;
+  uintptr_t tb;
+  std::memcpy(&tb, &tolbits, sizeof(tb));
+  const std::vector<uintptr_t> key = {
+      reinterpret_cast<uintptr_t>(thickness), reinterpret_cast<uintptr_t>(vp), reinterpret_cast<uintptr_t>(vs),
+      reinterpret_cast<uintptr_t>(rho), static_cast<uintptr_t>(B), reinterpret_cast<uintptr_t>(periods),
+      static_cast<uintptr_t>(row_periods), reinterpret_cast<uintptr_t>(bounds), tb, static_cast<uintptr_t>(wave),
+      reinterpret_cast<uintptr_t>(velocity), reinterpret_cast<uintptr_t>(flags), reinterpret_cast<uintptr_t>(levels),
+      reinterpret_cast<uintptr_t>(counts), reinterpret_cast<uintptr_t>(group), reinterpret_cast<uintptr_t>(group_flags),
+      static_cast<uintptr_t>(s->cfg.scan_waves), static_cast<uintptr_t>(s->cfg.consumer_blocks)};
+  if (s->cfg.graph && s->graph_exec && key == s->graph_key) {
+    MD_HIP(hipEventRecord(s->ev_begin, s->stream));
+    MD_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+    MD_HIP(hipEventRecord(s->ev_end, s->stream));
+    s->marked_chunks = 0;
+    return 0;
+  }
   MD_HIP(hipEventRecord(s->ev_begin, s->stream));
-  const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
-  MD_HIP(mdr::launch_prep(thickness, vp, vs, rho, bounds, periods, row_periods, B, L, T, s->nmodes, pp,
-                          s->rows.as<mdr::PrepRow>(), s->models.as<mdr::ModelInfo>(), s->traces.as<mdr::TraceInfo>(),
-                          s->stream));
-  if (int e = run_pipeline(s, ntraces, s->nmodes + 2, s->L, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
-                           s->cfg.group_velocity ? group : nullptr, group_flags))
+  if (int e = device_sequence(s, thickness, vp, vs, rho, B, periods, row_periods, bounds, c_tol_rel, wave, velocity,
+                              flags, levels, counts, group, group_flags))
     return e;
   MD_HIP(hipEventRecord(s->ev_end, s->stream));
+  if (s->cfg.graph) {
+    // Record the same sequence (every buffer is now allocated) on the solver's
+    // own stream; later calls with these arguments replay it.
+    if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    s->graph_key.clear();
+    hipStream_t issue = s->stream;
+    s->stream = s->own_stream;
+    s->capturing = true;
+    hipError_t err = hipStreamBeginCapture(s->own_stream, hipStreamCaptureModeRelaxed);
+    int e = err == hipSuccess ? device_sequence(s, thickness, vp, vs, rho, B, periods, row_periods, bounds, c_tol_rel,
+                                                wave, velocity, flags, levels, counts, group, group_flags)
+                              : 1;
+    hipGraph_t g = nullptr;
+    const hipError_t end = err == hipSuccess ? hipStreamEndCapture(s->own_stream, &g) : err;
+    s->capturing = false;
+    s->stream = issue;
+    if (e) return e;
+    if (end != hipSuccess) return fail(1, std::string("graph capture: ") + hipGetErrorString(end));
+    s->graph = g;
+    MD_HIP(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+    s->graph_key = key;
+  }
+  return 0;
+}
+
+int md_rocm_autotune(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                     const double *rho, int B, const double *periods, int row_periods, const double *bounds,
+                     double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
+                     int32_t *counts, double *group, uint32_t *group_flags, int *scan_waves,
+                     int *consumer_blocks) {
+  if (!s) return fail(2, "null solver");
+  const md_rocm_config saved = s->cfg;
+  s->cfg.graph = 0;
+  const int waves[] = {1, 2, 4, 8};
+  const int blocks[] = {4, 8, 16};
+  double best = 1e300;
+  int bw = 4, bb = 8;
+  for (int w : waves) {
+    if (w == 8 && !s->cfg.single_precision) continue;
+    for (int b : blocks) {
+      s->cfg.scan_waves = w;
+      s->cfg.consumer_blocks = b;
+      double ms = 0.;
+      for (int rep = 0; rep < 2; ++rep) {  // the first pass warms up, the second is timed
+        if (int e = md_rocm_run_device(s, thickness, vp, vs, rho, B, periods, row_periods, bounds, c_tol_rel, wave,
+                                       velocity, flags, levels, counts, group, group_flags)) {
+          s->cfg = saved;
+          return e;
+        }
+        MD_HIP(hipStreamSynchronize(s->stream));
+        float t = 0.f;
+        MD_HIP(hipEventElapsedTime(&t, s->ev_begin, s->ev_end));
+        ms = t;
+      }
+      if (ms < best) { best = ms; bw = w; bb = b; }
+    }
+  }
+  s->cfg = saved;
+  s->cfg.scan_waves = bw;
+  s->cfg.consumer_blocks = bb;
+  if (scan_waves) *scan_waves = bw;
+  if (consumer_blocks) *consumer_blocks = bb;
   return 0;
 }
 
