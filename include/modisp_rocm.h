@@ -38,7 +38,11 @@ typedef struct md_rocm_config {
   int single_precision;    /* 1: FP32 grid scan, FP64 recheck/refine; 0: FP64 grid scan */
   int group_velocity;      /* 1: analytic FP64 group velocity */
   double group_rtol;       /* accepted for API parity; the analytic derivative is exact */
-  int continuation;        /* anchor continuation across periods (0 = independent) */
+  int continuation;        /* 1: anchor continuation across periods on the host-array entry points
+                              (md_rocm_run, md_rocm_run_ragged): anchors every <= anchor_stride periods
+                              (log step <= 0.25) by the full pipeline, the rest predicted, refined and
+                              audited, failures re-solved by the full pipeline; models with fewer than
+                              8 strictly increasing periods are solved independently */
   int anchor_stride;       /* anchors every <= stride periods, default 4 */
   int nc;                  /* uniform grid: samples per trace, 3..65536 (default 512) */
   double tau_floor;        /* FP32 |F| uncertainty band floor (default 1e-4) */
@@ -135,7 +139,7 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
                        const double *rho, const int64_t *row_offsets, int B, const double *periods,
                        const int64_t *period_offsets, const int32_t *modes, const double *bounds,
                        double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
-                       int32_t *counts, double *group, uint32_t *group_flags);
+                       int32_t *counts, double *group, uint32_t *group_flags, uint32_t *prediction_flags);
 
 /* Wait for the solver's stream; returns the first asynchronous error. */
 int md_rocm_synchronize(md_rocm_solver *s);

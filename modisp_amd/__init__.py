@@ -31,7 +31,7 @@ SUPPORTS_AUTO_ACCURACY = True
 SUPPORTS_CANONICAL_LOWER = True
 SUPPORTS_ROW_PERIODS = True
 SUPPORTS_GROUP_VELOCITY = True
-SUPPORTS_CONTINUATION = False
+SUPPORTS_CONTINUATION = True
 STAT_NAMES = ("traces", "fp64_rechecks", "pair_probes", "fp64_rescans", "love_phase_searches", "scan_nodes")
 
 __all__ = ["load", "QeSolver", "available", "set_device", "memory_info", "device_name", "bounds",
@@ -126,8 +126,6 @@ class QeSolver:
             raise ValueError("grid must be adaptive or uniform")
         if scan not in ("float32", "float64"):
             raise ValueError("scan must be float32 or float64")
-        if continuation:
-            raise NotImplementedError("native anchor continuation is not available in this plugin build")
         cfg = _native.Config()
         L_ = _native.lib()
         L_.md_rocm_default_config(ctypes.byref(cfg))
@@ -153,6 +151,7 @@ class QeSolver:
         self.single_precision = bool(single_precision)
         self.scan = scan
         self.group_velocity = bool(group_velocity)
+        self.continuation = bool(continuation)
         self.config = cfg
         handle = L_.md_rocm_create(self.capacity, self.L, self.nf, self.nmodes, ctypes.byref(cfg))
         if not handle:
@@ -191,17 +190,19 @@ class QeSolver:
         gv = np.empty((B, M, T)) if self.group_velocity else None
         gf = np.empty((B, M, T), dtype=np.uint32) if self.group_velocity else None
         stats = np.zeros(len(STAT_NAMES), dtype=np.uint64)
+        pred = np.empty((B, T), dtype=np.uint32) if self.continuation else None
         code = _native.lib().md_rocm_run(
             self.handle, _native.ptr(h), _native.ptr(a), _native.ptr(b), _native.ptr(r), B, _native.ptr(per), row,
             _native.ptr(bnd), float(c_tol_rel), float(f_accept), int(wave), int(bool(automatic)),
             vel.ctypes.data_as(ctypes.c_void_p), int(self.single_precision), _native.ptr(flags, _native._u32p),
             _native.ptr(levels, _native._i32p), _native.ptr(counts, _native._i32p), _native.ptr(gv),
-            _native.ptr(gf, _native._u32p), None, _native.ptr(stats, _native._u64p), len(STAT_NAMES))
+            _native.ptr(gf, _native._u32p), _native.ptr(pred, _native._u32p), _native.ptr(stats, _native._u64p),
+            len(STAT_NAMES))
         _native.check(code)
         st = {k: int(v) for k, v in zip(STAT_NAMES, stats)}
         if profile:
             st["timings_s"] = {"device_total": self.last_timing()[0] * 1e-3}
-        extras = {"group_velocity": gv, "group_velocity_flags": gf, "prediction_flags": None}
+        extras = {"group_velocity": gv, "group_velocity_flags": gf, "prediction_flags": pred}
         return vel, flags, st, levels, counts, extras
 
     def run_ragged(self, thickness, vp, vs, rho, row_offsets, periods, period_offsets, modes, bounds, c_tol_rel,
@@ -223,15 +224,16 @@ class QeSolver:
         counts = np.empty(ntr, dtype=np.int32)
         gv = np.empty(ncell) if self.group_velocity else None
         gf = np.empty(ncell, dtype=np.uint32) if self.group_velocity else None
+        pred = np.empty(ntr, dtype=np.uint32)
         code = _native.lib().md_rocm_run_ragged(
             self.handle, _native.ptr(h), _native.ptr(a), _native.ptr(b), _native.ptr(r), _native.ptr(ro, _native._i64p),
             B, _native.ptr(per), _native.ptr(po, _native._i64p), _native.ptr(md, _native._i32p), _native.ptr(bnd),
             float(c_tol_rel), int(wave), _native.ptr(vel), _native.ptr(flags, _native._u32p),
             _native.ptr(levels, _native._i32p), _native.ptr(counts, _native._i32p), _native.ptr(gv),
-            _native.ptr(gf, _native._u32p))
+            _native.ptr(gf, _native._u32p), _native.ptr(pred, _native._u32p))
         _native.check(code)
         return dict(velocity=vel, flags=flags, levels=levels, counts=counts, group_velocity=gv,
-                    group_velocity_flags=gf)
+                    group_velocity_flags=gf, prediction_flags=pred)
 
     def run_device_batch(self, B, pointers, c_tol_rel, wave, row_periods=False):
         """pointers: dict of device addresses (int) for thickness, vp, vs, rho,

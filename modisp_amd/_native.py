@@ -76,7 +76,7 @@ def lib():
                                      ctypes.c_double, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]
     L.md_rocm_run_device.restype = ctypes.c_int
     L.md_rocm_run_ragged.argtypes = [_vp, _dp, _dp, _dp, _dp, _i64p, ctypes.c_int, _dp, _i64p, _i32p, _dp,
-                                     ctypes.c_double, ctypes.c_int, _dp, _u32p, _i32p, _i32p, _dp, _u32p]
+                                     ctypes.c_double, ctypes.c_int, _dp, _u32p, _i32p, _i32p, _dp, _u32p, _u32p]
     L.md_rocm_run_ragged.restype = ctypes.c_int
     L.md_rocm_synchronize.argtypes = [_vp]
     L.md_rocm_synchronize.restype = ctypes.c_int

@@ -894,6 +894,10 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(PipeArgs a) {
     if (a.velocity32) a.velocity32[cell] = float(r);
   }
   (void)nfinite;
+  if (a.all_roots) {
+    double *dst = a.all_roots + static_cast<size_t>(t) * a.kmax;
+    for (int m = 0; m < a.kmax; ++m) dst[m] = m < count ? roots[m] : NAN;
+  }
   if (cutoff) flags |= kFlagNearCutoff;
   if (close) flags |= kFlagCloseRoot;
   const size_t tc = static_cast<size_t>(mi.trace_out0) + ti.f;
@@ -970,6 +974,275 @@ __global__ __launch_bounds__(kBlock) void group_kernel(PipeArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Anchor continuation (the step rules follow cpp/common/continuation.hpp:12-212).
+// ---------------------------------------------------------------------------
+// Anchor slopes dc/dlnT = c (c/U - 1) from the analytic group velocity U.
+__global__ __launch_bounds__(kBlock) void slope_kernel(ContArgs a) {
+  const int total = a.nanchors * a.kmax;
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < total; q += gridDim.x * kBlock) {
+    const int p = q / a.kmax;
+    const double c = a.anchor_roots[q];
+    double slope = NAN;
+    if (c == c) {
+      const TraceInfo ti = a.traces[a.anchors[p]];
+      const ModelInfo mi = a.models[ti.model];
+      const double u = group_at(a.rows + mi.row0, mi.L, mi.top, a.wave, ti.omega, c);
+      if (u == u && fabs(u) < INFINITY && u != 0.) slope = c * (c / u - 1.);
+    }
+    a.anchor_slopes[q] = slope;
+  }
+}
+
+__global__ void anchor_flag_kernel(ContArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.nanchors) return;
+  const TraceInfo ti = a.traces[a.anchors[p]];
+  a.prediction[a.models[ti.model].trace_out0 + ti.f] = kPredAnchor;
+}
+
+// Anchor roots usable as a continuation source (continuation.hpp:22-31).
+__device__ bool cont_source_safe(const double *roots, int nr, int K, unsigned flags, double close_rtol) {
+  constexpr unsigned unusable = 4u | 8u | 16u | 32u | 64u | 128u | 512u | 1024u;
+  if (nr < 1 || nr > K || (flags & unusable)) return false;
+  for (int j = 0; j < nr; ++j) {
+    if (!(roots[j] > 0.) || !(roots[j] < INFINITY)) return false;
+    if (j && roots[j] - roots[j - 1] <= close_rtol * roots[j]) return false;
+  }
+  return true;
+}
+
+__device__ double cont_cap(const double *pred, int j, int nr, double lower, double upper) {
+  const double c = pred[j];
+  double gap = fmin(c - lower, upper - c);
+  if (j) gap = fmin(gap, c - pred[j - 1]);
+  if (j + 1 < nr) gap = fmin(gap, pred[j + 1] - c);
+  return fmin(.02 * c, .2 * gap);
+}
+
+// Cubic Hermite prediction in ln T from the two anchors (continuation.hpp:42-71).
+__device__ unsigned cont_predict(const double *cl0, const double *cr0, const double *sl0, const double *sr0, int nr,
+                                 double dl, double dr, double lower, double upper, double tol, double *pred,
+                                 double *dis) {
+  if (!(dl > 0. && dr < 0. && upper > lower)) return kPredWindow;
+  const double span = dl - dr, u = dl / span, u2 = u * u, u3 = u2 * u;
+  double last_l = 0., last_r = 0., last_up = 0.;
+  for (int j = 0; j < nr; ++j) {
+    const double c0 = cl0[j], c1 = cr0[j], sl = sl0[j], sr = sr0[j];
+    if (!(fabs(sl) < INFINITY) || !(fabs(sr) < INFINITY)) return kPredAnchorUnsafe;
+    const double pl = c0 + sl * dl, pr = c1 + sr * dr;
+    const double c = (2. * u3 - 3. * u2 + 1.) * c0 + (u3 - 2. * u2 + u) * span * sl + (-2. * u3 + 3. * u2) * c1 +
+                     (u3 - u2) * span * sr;
+    if (!(fabs(c) < INFINITY) || !(fabs(pl) < INFINITY) || !(fabs(pr) < INFINITY) || !(c > lower && c < upper))
+      return kPredWindow;
+    const double plo = fmin(pl, pr), phi = fmax(pl, pr);
+    if (j && (c <= pred[j - 1] || pl <= last_l || pr <= last_r || plo <= last_up)) return kPredWindow | kPredReview;
+    last_l = pl;
+    last_r = pr;
+    last_up = phi;
+    pred[j] = c;
+    dis[j] = fmax(fabs(pl - pr), fabs(c - ((1. - u) * c0 + u * c1)));
+  }
+  for (int j = 0; j < nr; ++j) {
+    const double cap = cont_cap(pred, j, nr, lower, upper);
+    if (!(cap > 16. * tol * pred[j]) || dis[j] > .5 * cap) return kPredWindow;
+  }
+  return 0u;
+}
+
+// Golden-section search for a point of the opposite sign between two
+// same-sign samples (an even root pair); true if one is found.
+__device__ bool cont_hidden_pair(const PrepRow *rows, int L, int top, int wave, double omega, double lo, double hi,
+                                 double sign) {
+  constexpr double phi = 0.6180339887498949;
+  double p = hi - phi * (hi - lo), r = lo + phi * (hi - lo);
+  double fp = sign * secular_rows(rows, L, top, wave, omega, p), fr = sign * secular_rows(rows, L, top, wave, omega, r);
+  for (int it = 0; it < 40; ++it) {
+    if (fp <= 0. || fr <= 0.) return true;
+    if (fp < fr) { hi = r; r = p; fr = fp; p = hi - phi * (hi - lo); fp = sign * secular_rows(rows, L, top, wave, omega, p); }
+    else { lo = p; p = r; fp = fr; r = lo + phi * (hi - lo); fr = sign * secular_rows(rows, L, top, wave, omega, r); }
+  }
+  return false;
+}
+
+// Audit of one refined root's bracket (continuation.hpp:126-166): Love by the
+// phase levels at the bracket ends; Rayleigh by three samples on each open
+// side of the root, with a pair search at a same-sign trough.
+__device__ unsigned cont_window(const PrepRow *rows, int L, int top, int wave, double omega, double a, double b,
+                                double fa, double fb, double root, double tol, double level) {
+  if (wave) {
+    constexpr double pi = 3.141592653589793;
+    const double left = love_phase_dev(rows, L, top, omega, a), right = love_phase_dev(rows, L, top, omega, b);
+    if (!(left == left) || !(right == right)) return kPredAuditFailed;
+    if (floor(left / pi) != level || floor(right / pi) != level - 1.) return kPredAuditFailed | kPredReview;
+    return 0u;
+  }
+  const double margin = fmax(tol, 4. * 2.220446049250313e-16) * root;
+  for (int side = 0; side < 2; ++side) {
+    const double edge = side ? b : a, sign = (side ? fb : fa) < 0. ? -1. : 1.;
+    if (fabs(root - edge) <= 2. * margin) continue;
+    const double end = root + copysign(margin, edge - root), step = (end - edge) * .25;
+    double mag[3];
+    for (int k = 0; k < 3; ++k) {
+      const double f = sign * secular_rows(rows, L, top, wave, omega, edge + (k + 1) * step);
+      if (!(f == f)) return kPredAuditFailed;
+      if (f <= 0.) return kPredAuditFailed | kPredReview;
+      mag[k] = f;
+    }
+    if (mag[1] < mag[0] && mag[1] < mag[2] && (mag[1] < .5 * fmin(mag[0], mag[2]) || mag[1] < .05)) {
+      const double x = edge + step, y = edge + 3. * step;
+      if (cont_hidden_pair(rows, L, top, wave, omega, fmin(x, y), fmax(x, y), sign)) return kPredAuditFailed | kPredReview;
+    }
+  }
+  return 0u;
+}
+
+// The gaps between the accepted brackets must hold no further root
+// (continuation.hpp:85-124): equal end signs, then samples spaced by the WKB
+// mode count (at least 4 per gap) and at every layer speed inside the gap,
+// none of opposite sign and none a trough that hides a pair.
+__device__ unsigned cont_gaps(const PrepRow *rows, int L, int top, int wave, double omega, double lower,
+                              double upper, const double *lo, const double *hi, int nr, int K) {
+  if (nr < 1 || nr > K) return kPredAuditFailed;
+  if (nr == K) upper = hi[nr - 1];
+  auto density = [&](double c) {
+    const double k = omega / c, k2 = k * k;
+    double n = 0.;
+    for (int j = top; j < L - 1; ++j) {
+      const double kb = omega * rows[j].ivs, ka = omega * rows[j].ivp;
+      n += rows[j].h * sqrt(fmax(0., kb * kb - k2));
+      if (wave == 0) n += rows[j].h * sqrt(fmax(0., ka * ka - k2));
+    }
+    return n * 0.3183098861837907;
+  };
+  for (int g = 0; g < nr + (nr < K); ++g) {
+    const double a = g ? hi[g - 1] : lower, b = g < nr ? lo[g] : upper;
+    if (!(b > a)) continue;
+    const double fa = secular_rows(rows, L, top, wave, omega, a), fb = secular_rows(rows, L, top, wave, omega, b);
+    if (!(fa == fa) || !(fb == fb)) return kPredAuditFailed;
+    if (fa == 0. || fb == 0. || signbit(fa) != signbit(fb)) return kPredAuditFailed | kPredReview;
+    const double sign = fa < 0. ? -1. : 1., endpoint = fmin(fabs(fa), fabs(fb));
+    auto sample = [&](double c) -> unsigned {
+      const double f = secular_rows(rows, L, top, wave, omega, c);
+      if (!(f == f)) return kPredAuditFailed;
+      if (f == 0. || sign * f < 0. || (fabs(f) < .05 && fabs(f) < .1 * endpoint)) return kPredAuditFailed | kPredReview;
+      return 0u;
+    };
+    const int n = max(4, min(64, int(ceil(8. * (density(b) - density(a))))));
+    for (int i = 1; i <= n; ++i)
+      if (unsigned issue = sample(a + (b - a) * double(i) / double(n + 1))) return issue;
+    for (int j = 0; j < L; ++j) {
+      const double v[2] = {rows[j].ivs > 0. ? 1. / rows[j].ivs : 0., 1. / rows[j].ivp};
+      for (int body = 0; body < 2; ++body)
+        for (int side = -1; side <= 1; ++side) {
+          const double c = v[body] * (1. + side * 1e-7);
+          if (c > a && c < b)
+            if (unsigned issue = sample(c)) return issue;
+        }
+    }
+  }
+  return 0u;
+}
+
+// One lane per target trace.
+__global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < a.ntargets; q += gridDim.x * kBlock) {
+    const TargetPlan tp = a.targets[q];
+    const TraceInfo ti = a.traces[tp.trace];
+    const ModelInfo mi = a.models[ti.model];
+    const PrepRow *rows = a.rows + mi.row0;
+    const TraceInfo tl = a.traces[a.anchors[tp.left]], tr = a.traces[a.anchors[tp.right]];
+    const long long cl = mi.trace_out0 + tl.f, cr = mi.trace_out0 + tr.f, ct = mi.trace_out0 + ti.f;
+    const int nr = a.counts[cl], K = mi.K;
+    unsigned pred = ((a.flags[cl] | a.flags[cr]) & kFlagCloseRoot) ? kPredSourceUncertain : 0u;
+    unsigned reason = 0u;
+    const double *rl = a.anchor_roots + static_cast<size_t>(tp.left) * a.kmax;
+    const double *rr = a.anchor_roots + static_cast<size_t>(tp.right) * a.kmax;
+    double pv[kContMax], dis[kContMax], lo[kContMax], hi[kContMax], roots[kContMax];
+    if (mi.empty) reason = kPredWindow;
+    else if (nr <= 0 || nr > K || nr > kContMax || a.counts[cr] != nr) reason = kPredRootCount;
+    else if (!cont_source_safe(rl, nr, K, a.flags[cl], a.close_rtol) ||
+             !cont_source_safe(rr, nr, K, a.flags[cr], a.close_rtol))
+      reason = kPredAnchorUnsafe;
+    else
+      reason = cont_predict(rl, rr, a.anchor_slopes + static_cast<size_t>(tp.left) * a.kmax,
+                            a.anchor_slopes + static_cast<size_t>(tp.right) * a.kmax, nr, tp.dl, tp.dr, mi.lo, mi.hi,
+                            a.tol, pv, dis);
+    if (!reason) {
+      // Local brackets around the predictions, widened up to three times
+      // (continuation.hpp:170-212), refined in FP64, then audited.
+      double first_level = 0.;
+      if (a.wave) {
+        const double ph = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.lo);
+        if (!(ph == ph)) reason = kPredAuditFailed;
+        first_level = floor(ph / 3.141592653589793);
+      }
+      for (int j = 0; j < nr && !reason; ++j) {
+        const double c = pv[j], cap = cont_cap(pv, j, nr, mi.lo, mi.hi);
+        double width = fmin(cap, fmax(16. * a.tol * c, fmax(2. * dis[j], .05 * c * fmin(fabs(tp.dl), fabs(tp.dr)))));
+        double x0 = 0., x1 = 0., f0 = 0., f1 = 0.;
+        bool bracket = false;
+        for (int attempt = 0; attempt < 4; ++attempt) {
+          x0 = fmax(mi.lo, c - width);
+          x1 = fmin(mi.hi, c + width);
+          f0 = secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, x0);
+          f1 = secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, x1);
+          if (f0 == f0 && f1 == f1 && f0 != 0. && f1 != 0. && signbit(f0) != signbit(f1)) { bracket = true; break; }
+          if (width >= cap) break;
+          width = fmin(cap, 2. * width);
+        }
+        if (!bracket) { reason = kPredBracket; break; }
+        int ev = 0;
+        const double root = brent(rows, mi.L, mi.top, a.wave, ti.omega, x0, x1, f0, f1, a.tol, ev);
+        if (!(root >= x0 && root <= x1)) { reason = kPredCorrection; break; }
+        if (j && root - roots[j - 1] <= a.close_rtol * fabs(root)) { reason = kPredCorrection | kPredReview; break; }
+        const unsigned issue = cont_window(rows, mi.L, mi.top, a.wave, ti.omega, x0, x1, f0, f1, root, a.tol,
+                                           first_level - j);
+        if (issue) { reason = issue; pred |= kPredAudit; break; }
+        lo[j] = x0;
+        hi[j] = x1;
+        roots[j] = root;
+      }
+      if (!reason) {
+        pred |= kPredAudit;
+        reason = cont_gaps(rows, mi.L, mi.top, a.wave, ti.omega, mi.lo, mi.hi, lo, hi, nr, K);
+      }
+    }
+    if (reason) {
+      a.prediction[ct] = pred | reason | kPredFallback;
+      a.fallback[atomicAdd(a.fallback_count, 1)] = tp.trace;
+      continue;
+    }
+    a.prediction[ct] = pred | kPredAccepted;
+    unsigned word = kFlagApproximate | kFlagContinuation;
+    for (int m = 0; m < mi.modes; ++m) {
+      const double c = m < nr ? roots[m] : NAN;
+      if (c == c && c >= .99 * mi.vs_half) word |= kFlagNearCutoff;
+      const size_t cell = static_cast<size_t>(mi.out0) + static_cast<size_t>(m) * mi.T + ti.f;
+      a.velocity[cell] = c;
+      if (a.group) {
+        double u = NAN;
+        unsigned gf = 0;
+        if (!(c == c)) gf = kGroupMissing;
+        else {
+          u = group_at(rows, mi.L, mi.top, a.wave, ti.omega, c);
+          if (!(u == u) || fabs(u) == INFINITY) { u = NAN; gf = kGroupNonfinite; }
+        }
+        a.group[cell] = u;
+        a.group_flags[cell] = gf;
+      }
+    }
+    a.flags[ct] = word;
+    a.levels[ct] = kLevelContinuation;
+    a.counts[ct] = nr;
+  }
+}
+
+__global__ void gather_kernel(const TraceInfo *all, const int *list, int n, TraceInfo *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = all[list[i]];
+}
+
 // Reset the device work counters (a kernel, so the sequence can be graph-captured).
 __global__ void reset_kernel(int *counters, int n) {
   if (threadIdx.x < n) counters[threadIdx.x] = 0;
@@ -1037,6 +1310,20 @@ static hipError_t launch_scan_t(const PipeArgs &a, hipStream_t s) {
   } else {
     launch_scan_w<T, 4>(a, n, s);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_continuation(const ContArgs &c, int persistent_blocks, hipStream_t s) {
+  if (c.nanchors > 0) {
+    hipLaunchKernelGGL(slope_kernel, dim3(persistent_blocks), dim3(kBlock), 0, s, c);
+    hipLaunchKernelGGL(anchor_flag_kernel, dim3((c.nanchors + 255) / 256), dim3(256), 0, s, c);
+  }
+  if (c.ntargets > 0) hipLaunchKernelGGL(continuation_kernel, dim3(persistent_blocks), dim3(kBlock), 0, s, c);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const TraceInfo *all, const int *list, int n, TraceInfo *out, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(gather_kernel, dim3((n + 255) / 256), dim3(256), 0, s, all, list, n, out);
   return hipGetLastError();
 }
 

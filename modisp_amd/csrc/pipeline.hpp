@@ -144,7 +144,51 @@ struct PipeArgs {
   double *group;
   uint32_t *group_flags;
   unsigned long long *stats;  // optional run statistics (see stats_kernel)
+  double *all_roots;  // optional: every (up to kmax) root per listed trace, [trace list position][kmax]
 };
+
+// ---------------------------------------------------------------------------
+// Anchor continuation across periods (reference cpp/common/continuation.hpp,
+// cpp/cpu/solver.cpp:605-705): anchors are solved by the full pipeline; every
+// other period of the model is predicted from its two anchors (cubic Hermite
+// in ln T with slopes dc/dlnT = c (c/U - 1)), bracketed locally, refined in
+// FP64 and audited; a target that fails falls back to the full pipeline.
+// ---------------------------------------------------------------------------
+constexpr unsigned kPredAnchor = 1u, kPredAccepted = 2u, kPredFallback = 4u, kPredBypassed = 8u,
+                   kPredAudit = 16u, kPredAuditFailed = 32u, kPredSourceUncertain = 64u, kPredRootCount = 128u,
+                   kPredAnchorUnsafe = 256u, kPredWindow = 512u, kPredBracket = 1024u, kPredCorrection = 2048u,
+                   kPredReview = 4096u;
+constexpr unsigned kFlagApproximate = 2048u, kFlagContinuation = 65536u;
+constexpr int kLevelContinuation = 12;
+constexpr int kContMax = 16;  // roots per trace the continuation kernel handles (K = modes + 2)
+
+struct TargetPlan {
+  int trace;      // index in the full trace array
+  int left, right;  // anchor positions (into the anchor list)
+  double dl, dr;  // log steps from the target period to the left / right anchor periods
+};
+
+struct ContArgs {
+  const TraceInfo *traces;   // full trace array
+  const ModelInfo *models;
+  const PrepRow *rows;
+  const int *anchors;        // anchor list: trace indices
+  const TargetPlan *targets;
+  int ntargets, nanchors, kmax, wave;
+  const double *anchor_roots;  // [anchor][kmax]
+  double *anchor_slopes;       // [anchor][kmax]
+  double tol, close_rtol;
+  double *velocity;
+  uint32_t *flags;
+  int32_t *levels, *counts;
+  double *group;
+  uint32_t *group_flags;
+  uint32_t *prediction;        // [trace] (cell = model trace_out0 + f)
+  int *fallback;               // trace indices to solve with the full pipeline
+  int *fallback_count;
+};
+hipError_t launch_continuation(const ContArgs &c, int persistent_blocks, hipStream_t s);
+hipError_t launch_gather(const TraceInfo *all, const int *list, int n, TraceInfo *out, hipStream_t s);
 using ScanArgs = PipeArgs;
 
 // marks (optional): 7 events recorded before the scan and after the scan,

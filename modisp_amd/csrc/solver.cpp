@@ -72,6 +72,7 @@ struct md_rocm_solver {
   hipStream_t own_stream = nullptr;  // the solver's own stream
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats, overflow, phase, tape;
+  DevBuf c_anchors, c_targets, c_list, c_roots, c_slopes, c_fallback, c_count, c_pred;
   DevBuf in_h, in_vp, in_vs, in_rho, in_per, in_bounds;
   DevBuf out_v, out_flags, out_levels, out_counts, out_group, out_gflags;
   double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -81,10 +82,12 @@ struct md_rocm_solver {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<uintptr_t> graph_key;  // arguments the captured graph was built for
+  long long cont_accepted = 0, cont_fallbacks = 0;  // continuation targets of the last run
 
   ~md_rocm_solver() {
     for (DevBuf *b : {&rows, &models, &traces, &events, &scan, &brackets, &roots, &unc_q, &tr_q, &counters, &stats,
-                      &overflow, &phase, &tape,
+                      &overflow, &phase, &tape, &c_anchors, &c_targets, &c_list, &c_roots, &c_slopes,
+                      &c_fallback, &c_count, &c_pred,
                       &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &out_v, &out_flags, &out_levels,
                       &out_counts, &out_group, &out_gflags})
       b->release();
@@ -108,8 +111,8 @@ int chunk_size(const md_rocm_solver *s, long long ntraces) {
 
 // Runs the root pipeline over all traces already prepared on the device.
 int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, int wave, double tol, double *v64,
-                 float *v32,
-                 uint32_t *flags, int32_t *levels, int32_t *counts, double *group, uint32_t *gflags) {
+                 float *v32, uint32_t *flags, int32_t *levels, int32_t *counts, double *group, uint32_t *gflags,
+                 const mdr::TraceInfo *list = nullptr, double *all_roots = nullptr) {
   const int chunk = chunk_size(s, ntraces);
   const size_t qcap = static_cast<size_t>(chunk) * mdr::kEventCap;
   if (int e = s->events.ensure(qcap * sizeof(mdr::Event))) return e;
@@ -123,7 +126,8 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   if (int e = s->phase.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->stats.ensure(kStatCount * sizeof(unsigned long long))) return e;
   mdr::PipeArgs a{};
-  a.traces = s->traces.as<mdr::TraceInfo>();
+  a.traces = list ? list : s->traces.as<mdr::TraceInfo>();
+  a.all_roots = all_roots;
   a.models = s->models.as<mdr::ModelInfo>();
   a.rows = s->rows.as<mdr::PrepRow>();
   a.events = s->events.as<mdr::Event>();
@@ -173,6 +177,101 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
     const hipError_t err = mdr::launch_pipeline(a, a, a.fp32 != 0, group != nullptr, persistent, s->stream,
                                                 s->capturing ? nullptr : s->marks.data() + c * mdr::kStageMarks);
     if (err != hipSuccess) return fail(1, std::string("kernel launch: ") + hipGetErrorString(err));
+  }
+  return 0;
+}
+
+// Anchor / target plan of one model's periods (reference continuation.hpp:12-20,
+// cpu/continuation.hpp:13-24). Models with fewer than 8 periods or periods
+// that are not strictly increasing are solved independently (all anchors).
+double cont_log_step(double target, double anchor) {
+  if (std::fabs(target - anchor) < .5 * anchor) return std::log1p((target - anchor) / anchor);
+  return std::log(target) - std::log(anchor);
+}
+
+void plan_model(const double *p, int T, long long base, int stride, std::vector<int> &anchors,
+                std::vector<mdr::TargetPlan> &targets, bool &bypassed) {
+  bool ok = T >= 8;
+  for (int f = 1; f < T && ok; ++f) ok = p[f] > p[f - 1];
+  bypassed = !ok;
+  if (!ok) {
+    for (int f = 0; f < T; ++f) anchors.push_back(static_cast<int>(base + f));
+    return;
+  }
+  for (int left = 0;;) {
+    const int slot = static_cast<int>(anchors.size());
+    anchors.push_back(static_cast<int>(base + left));
+    if (left == T - 1) break;
+    int right = left + std::min(stride, T - 1 - left);
+    while (right > left + 1 && cont_log_step(p[right], p[left]) > .25) --right;
+    for (int f = left + 1; f < right; ++f)
+      targets.push_back(mdr::TargetPlan{static_cast<int>(base + f), slot, slot + 1, cont_log_step(p[f], p[left]),
+                                        cont_log_step(p[f], p[right])});
+    left = right;
+  }
+}
+
+// Anchors through the full pipeline, targets by continuation, failed targets
+// through the full pipeline again. Outputs and prediction flags are device
+// arrays in the run's layout; traces/models/rows are already on the device.
+int run_continuation(md_rocm_solver *s, int kmax, int max_rows, int wave, double tol, double *v64, uint32_t *flags,
+                     int32_t *levels, int32_t *counts, double *group, uint32_t *gflags, uint32_t *prediction,
+                     const std::vector<int> &anchors, const std::vector<mdr::TargetPlan> &targets) {
+  const int na = static_cast<int>(anchors.size()), nt = static_cast<int>(targets.size());
+  hipStream_t st = s->stream;
+  const size_t list_n = static_cast<size_t>(std::max(na, nt));
+  if (int e = s->c_anchors.ensure(std::max<size_t>(1, na) * sizeof(int))) return e;
+  if (int e = s->c_targets.ensure(std::max<size_t>(1, nt) * sizeof(mdr::TargetPlan))) return e;
+  if (int e = s->c_list.ensure(std::max<size_t>(1, list_n) * sizeof(mdr::TraceInfo))) return e;
+  if (int e = s->c_roots.ensure(std::max<size_t>(1, na) * kmax * sizeof(double))) return e;
+  if (int e = s->c_slopes.ensure(std::max<size_t>(1, na) * kmax * sizeof(double))) return e;
+  if (int e = s->c_fallback.ensure(std::max<size_t>(1, nt) * sizeof(int))) return e;
+  if (int e = s->c_count.ensure(sizeof(int))) return e;
+  MD_HIP(hipMemcpyAsync(s->c_anchors.p, anchors.data(), na * sizeof(int), hipMemcpyHostToDevice, st));
+  if (nt) MD_HIP(hipMemcpyAsync(s->c_targets.p, targets.data(), nt * sizeof(mdr::TargetPlan), hipMemcpyHostToDevice, st));
+  const int zero = 0;
+  MD_HIP(hipMemcpyAsync(s->c_count.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+  MD_HIP(mdr::launch_gather(s->traces.as<mdr::TraceInfo>(), s->c_anchors.as<int>(), na,
+                            s->c_list.as<mdr::TraceInfo>(), st));
+  if (int e = run_pipeline(s, na, kmax, max_rows, wave, tol, v64, nullptr, flags, levels, counts, group, gflags,
+                           s->c_list.as<mdr::TraceInfo>(), s->c_roots.as<double>()))
+    return e;
+  mdr::ContArgs c{};
+  c.traces = s->traces.as<mdr::TraceInfo>();
+  c.models = s->models.as<mdr::ModelInfo>();
+  c.rows = s->rows.as<mdr::PrepRow>();
+  c.anchors = s->c_anchors.as<int>();
+  c.targets = s->c_targets.as<mdr::TargetPlan>();
+  c.ntargets = nt;
+  c.nanchors = na;
+  c.kmax = kmax;
+  c.wave = wave;
+  c.anchor_roots = s->c_roots.as<double>();
+  c.anchor_slopes = s->c_slopes.as<double>();
+  c.tol = tol;
+  c.close_rtol = s->cfg.close_root_rtol;
+  c.velocity = v64;
+  c.flags = flags;
+  c.levels = levels;
+  c.counts = counts;
+  c.group = group;
+  c.group_flags = gflags;
+  c.prediction = prediction;
+  c.fallback = s->c_fallback.as<int>();
+  c.fallback_count = s->c_count.as<int>();
+  const int pb = s->cus * (s->cfg.consumer_blocks > 0 ? s->cfg.consumer_blocks : 8);
+  MD_HIP(mdr::launch_continuation(c, pb, st));
+  int nfb = 0;
+  MD_HIP(hipMemcpyAsync(&nfb, s->c_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  MD_HIP(hipStreamSynchronize(st));
+  s->cont_accepted = nt - nfb;
+  s->cont_fallbacks = nfb;
+  if (nfb > 0) {
+    MD_HIP(mdr::launch_gather(s->traces.as<mdr::TraceInfo>(), s->c_fallback.as<int>(), nfb,
+                              s->c_list.as<mdr::TraceInfo>(), st));
+    if (int e = run_pipeline(s, nfb, kmax, max_rows, wave, tol, v64, nullptr, flags, levels, counts, group, gflags,
+                             s->c_list.as<mdr::TraceInfo>(), nullptr))
+      return e;
   }
   return 0;
 }
@@ -509,13 +608,42 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
   MD_HIP(hipMemcpyAsync(s->in_per.p, periods, nper * sizeof(double), hipMemcpyHostToDevice, st));
   MD_HIP(hipMemcpyAsync(s->in_bounds.p, bounds, static_cast<size_t>(B) * 2 * sizeof(double), hipMemcpyHostToDevice,
                         st));
-  if (int e = md_rocm_run_device(s, s->in_h.as<double>(), s->in_vp.as<double>(), s->in_vs.as<double>(),
-                                 s->in_rho.as<double>(), B, s->in_per.as<double>(), row_periods,
-                                 s->in_bounds.as<double>(), c_tol_rel, wave, s->out_v.as<double>(),
-                                 s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(),
-                                 s->out_counts.as<int32_t>(), want_group ? s->out_group.as<double>() : nullptr,
-                                 want_group ? s->out_gflags.as<uint32_t>() : nullptr))
+  std::vector<char> bypassed;
+  if (s->cfg.continuation) {
+    if (int e = s->rows.ensure(nrow * sizeof(mdr::PrepRow))) return e;
+    if (int e = s->models.ensure(static_cast<size_t>(B) * sizeof(mdr::ModelInfo))) return e;
+    if (int e = s->traces.ensure(ntr * sizeof(mdr::TraceInfo))) return e;
+    if (int e = s->c_pred.ensure(ntr * sizeof(uint32_t))) return e;
+    MD_HIP(hipEventRecord(s->ev_begin, st));
+    const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
+    MD_HIP(mdr::launch_prep(s->in_h.as<double>(), s->in_vp.as<double>(), s->in_vs.as<double>(), s->in_rho.as<double>(),
+                            s->in_bounds.as<double>(), s->in_per.as<double>(), row_periods, B, L, T, M, pp,
+                            s->rows.as<mdr::PrepRow>(), s->models.as<mdr::ModelInfo>(),
+                            s->traces.as<mdr::TraceInfo>(), st));
+    std::vector<int> anchors;
+    std::vector<mdr::TargetPlan> targets;
+    bypassed.assign(B, 0);
+    for (int m = 0; m < B; ++m) {
+      bool by = false;
+      plan_model(row_periods ? periods + static_cast<size_t>(m) * T : periods, T, static_cast<long long>(m) * T,
+                 s->cfg.anchor_stride, anchors, targets, by);
+      bypassed[m] = by;
+    }
+    if (int e = run_continuation(s, M + 2, L, wave, c_tol_rel, s->out_v.as<double>(), s->out_flags.as<uint32_t>(),
+                                 s->out_levels.as<int32_t>(), s->out_counts.as<int32_t>(),
+                                 want_group ? s->out_group.as<double>() : nullptr,
+                                 want_group ? s->out_gflags.as<uint32_t>() : nullptr, s->c_pred.as<uint32_t>(),
+                                 anchors, targets))
+      return e;
+    MD_HIP(hipEventRecord(s->ev_end, st));
+  } else if (int e = md_rocm_run_device(s, s->in_h.as<double>(), s->in_vp.as<double>(), s->in_vs.as<double>(),
+                                        s->in_rho.as<double>(), B, s->in_per.as<double>(), row_periods,
+                                        s->in_bounds.as<double>(), c_tol_rel, wave, s->out_v.as<double>(),
+                                        s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(),
+                                        s->out_counts.as<int32_t>(), want_group ? s->out_group.as<double>() : nullptr,
+                                        want_group ? s->out_gflags.as<uint32_t>() : nullptr)) {
     return e;
+  }
   std::vector<double> vtmp;
   double *vdst = static_cast<double *>(velocity);
   if (velocity_is_f32) {
@@ -535,7 +663,15 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
     float *out = static_cast<float *>(velocity);
     for (size_t i = 0; i < ncell; ++i) out[i] = static_cast<float>(vtmp[i]);
   }
-  if (prediction_flags) std::fill_n(prediction_flags, ntr, 8u);  // bypassed: independent periods
+  if (prediction_flags) {
+    if (s->cfg.continuation) {
+      MD_HIP(hipMemcpy(prediction_flags, s->c_pred.p, ntr * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (int m = 0; m < B; ++m)
+        if (bypassed[m]) std::fill_n(prediction_flags + static_cast<size_t>(m) * T, T, mdr::kPredBypassed);
+    } else {
+      std::fill_n(prediction_flags, ntr, mdr::kPredBypassed);  // independent periods
+    }
+  }
   if (stats && nstats > 0) {
     unsigned long long st4[kStatCount] = {0, 0, 0, 0, 0, 0};
     MD_HIP(hipMemcpy(st4, s->stats.p, sizeof(st4), hipMemcpyDeviceToHost));
@@ -548,7 +684,7 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
                        const double *rho, const int64_t *row_offsets, int B, const double *periods,
                        const int64_t *period_offsets, const int32_t *modes, const double *bounds,
                        double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
-                       int32_t *counts, double *group, uint32_t *group_flags) {
+                       int32_t *counts, double *group, uint32_t *group_flags, uint32_t *prediction_flags) {
   if (!s) return fail(2, "null solver");
   if (B < 1) return fail(2, "B must be positive");
   if (!thickness || !vp || !vs || !rho || !row_offsets || !periods || !period_offsets || !modes || !bounds ||
@@ -597,11 +733,31 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
                         st));
   MD_HIP(hipMemcpyAsync(s->traces.p, traces.data(), traces.size() * sizeof(mdr::TraceInfo), hipMemcpyHostToDevice,
                         st));
-  if (int e = run_pipeline(s, ntr, kmax, max_rows, wave, c_tol_rel, s->out_v.as<double>(), nullptr,
-                           s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(), s->out_counts.as<int32_t>(),
-                           want_group ? s->out_group.as<double>() : nullptr,
-                           want_group ? s->out_gflags.as<uint32_t>() : nullptr))
+  std::vector<char> bypassed;
+  if (s->cfg.continuation) {
+    if (int e = s->c_pred.ensure(static_cast<size_t>(ntr) * sizeof(uint32_t))) return e;
+    std::vector<int> anchors;
+    std::vector<mdr::TargetPlan> targets;
+    bypassed.assign(B, 0);
+    for (int m = 0; m < B; ++m) {
+      bool by = false;
+      const long long p0 = period_offsets[m];
+      plan_model(periods + p0, static_cast<int>(period_offsets[m + 1] - p0), p0, s->cfg.anchor_stride, anchors,
+                 targets, by);
+      bypassed[m] = by;
+    }
+    if (int e = run_continuation(s, kmax, max_rows, wave, c_tol_rel, s->out_v.as<double>(),
+                                 s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(), s->out_counts.as<int32_t>(),
+                                 want_group ? s->out_group.as<double>() : nullptr,
+                                 want_group ? s->out_gflags.as<uint32_t>() : nullptr, s->c_pred.as<uint32_t>(),
+                                 anchors, targets))
+      return e;
+  } else if (int e = run_pipeline(s, ntr, kmax, max_rows, wave, c_tol_rel, s->out_v.as<double>(), nullptr,
+                                  s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(),
+                                  s->out_counts.as<int32_t>(), want_group ? s->out_group.as<double>() : nullptr,
+                                  want_group ? s->out_gflags.as<uint32_t>() : nullptr)) {
     return e;
+  }
   MD_HIP(hipEventRecord(s->ev_end, st));
   MD_HIP(hipMemcpyAsync(velocity, s->out_v.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
   MD_HIP(hipMemcpyAsync(flags, s->out_flags.p, static_cast<size_t>(ntr) * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -614,7 +770,18 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
     MD_HIP(hipMemcpyAsync(group, s->out_group.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
     MD_HIP(hipMemcpyAsync(group_flags, s->out_gflags.p, ncell * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   }
-  return md_rocm_synchronize(s);
+  if (prediction_flags) {
+    if (s->cfg.continuation)
+      MD_HIP(hipMemcpyAsync(prediction_flags, s->c_pred.p, static_cast<size_t>(ntr) * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, st));
+  }
+  if (int e = md_rocm_synchronize(s)) return e;
+  if (prediction_flags) {
+    for (int m = 0; m < B; ++m)
+      if (!s->cfg.continuation || bypassed[m])
+        std::fill(prediction_flags + period_offsets[m], prediction_flags + period_offsets[m + 1], mdr::kPredBypassed);
+  }
+  return 0;
 }
 
 }  // extern "C"
