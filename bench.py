@@ -18,6 +18,9 @@ solve + misfit loss + adjoint backward on the differentiation workload
 (4096 models x 21 rows, 50 periods, Rayleigh, 3 modes) through the
 torch.autograd binding.
 
+`uneven` times the ragged uneven batch (65536 models, per-model rows,
+periods and modes) with the independent search and with anchor continuation.
+
 `roofline` is for the dominant kernel, the FP32 Rayleigh grid scan: its
 algorithmic FLOP (secular evaluations x layer steps x 160 FLOP per Dunkin
 layer step, counted on the device) divided by its HIP-event time on the
@@ -206,6 +209,43 @@ def gradient_leg(torch, modisp_amd, workloads, steps, warmup):
                        "parameters": "h, vp, vs, rho of every row", "dtype": "f32 scan, f64 refine and adjoint"}}
 
 
+def uneven_leg(modisp_amd, workloads, models):
+    """The uneven batch (65536 models of 4..41 rows, 20..200 periods 0.2-100 s,
+    1..10 Rayleigh modes each) in one ragged call, independent search and anchor
+    continuation: device roots/s (HIP events around the whole call on the
+    solver stream; host packing and PCIe transfers of the ragged arrays are
+    included) and the continuation's acceptance."""
+    import numpy as np
+    wl = workloads.uneven_batch(B=models)
+    lay = wl.layers
+    row_off = np.concatenate([[0], np.cumsum(lay)]).astype(np.int64)
+    cols = [np.concatenate([a[i, :lay[i]] for i in range(wl.B)]) for a in (wl.thickness, wl.vp, wl.vs, wl.rho)]
+    per = np.concatenate(wl.periods)
+    per_off = np.concatenate([[0], np.cumsum([p.size for p in wl.periods])]).astype(np.int64)
+    bounds = np.array([[0.8 * wl.vs[i, :lay[i]].min(), np.nextafter(wl.vs[i, lay[i] - 1], 0.)] for i in range(wl.B)])
+    out = {}
+    for cont in (False, True):
+        with modisp_amd.QeSolver(wl.B, int(lay.max()), 1, 1, continuation=cont) as s:
+            s.run_ragged(*cols, row_off, per, per_off, wl.nmodes, bounds, 1e-12, 0)      # warm-up
+            t0 = time.perf_counter()
+            r = s.run_ragged(*cols, row_off, per, per_off, wl.nmodes, bounds, 1e-12, 0)
+            wall = time.perf_counter() - t0
+            dev_ms = float(s.last_timing()[0])
+        roots = int(np.isfinite(r["velocity"]).sum())
+        pred = r["prediction_flags"]
+        key = "continuation" if cont else "independent"
+        out[key] = {"value": roots / (dev_ms * 1e-3), "unit": "roots/s", "device_ms": dev_ms, "wall_ms": wall * 1e3,
+                    "roots": roots}
+        if cont:
+            targets = int(np.count_nonzero((pred & 1) == 0))
+            out[key]["targets"] = targets
+            out[key]["accepted"] = int(np.count_nonzero(pred & 2))
+            out[key]["fallbacks"] = int(np.count_nonzero(pred & 4))
+    out["config"] = {"workload": f"uneven: {wl.B} models, {int(per_off[-1])} traces, rows 4..41, periods 20..200, "
+                                 "modes 1..10, Rayleigh, 20% low-velocity zones", "entry_point": "md_rocm_run_ragged"}
+    return out
+
+
 def traffic_per_launch():
     """HBM bytes per scan launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (scaled by a copy-kernel calibration), if present."""
@@ -226,6 +266,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uniform", action="store_true", help="skip the 512-sample uniform-grid comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the ragged uneven-batch leg")
+    ap.add_argument("--uneven-models", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every trace is independent, "
@@ -318,6 +360,12 @@ def main():
         result["fwd_adjoint"] = gradient_leg(torch, modisp_amd, workloads, args.steps, args.warmup)
         log(f"fwd+adjoint: {result['fwd_adjoint']['ms_per_step']:.2f} ms/step, "
             f"{result['fwd_adjoint']['value']:.0f} models/s")
+    if not args.no_uneven:
+        result["uneven"] = uneven_leg(modisp_amd, workloads, args.uneven_models)
+        u = result["uneven"]
+        log(f"uneven: independent {u['independent']['device_ms']:.1f} ms, continuation "
+            f"{u['continuation']['device_ms']:.1f} ms ({u['continuation']['accepted']} of "
+            f"{u['continuation']['targets']} targets accepted)")
     if not args.no_cpu:
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
         result["cpu_baseline"] = cpu_baseline(wl, window, fast, threads, min(args.cpu_sample, wl.B))

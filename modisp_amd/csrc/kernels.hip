@@ -909,7 +909,7 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(PipeArgs a) {
 // ---------------------------------------------------------------------------
 // Analytic group velocity U = dw/dk = -F_k / F_w at each refined root.
 // ---------------------------------------------------------------------------
-__device__ double group_at(const PrepRow *rows, int L, int top, int wave, double omega, double c) {
+__device__ __forceinline__ double group_at(const PrepRow *rows, int L, int top, int wave, double omega, double c) {
   const double iw = 1. / omega, iw2 = iw * iw;
   const Dual2 k(omega / c, 1., 0.), w(omega, 0., 1.);
   auto layer = [&](const PrepRow &r) {

@@ -257,14 +257,42 @@ MDR_HD void compound_apply_transpose(const Compound<T> &q, T l[5]) {
   l[4] = q.O * l0 + q.F * l1 + q.J * l2 + q.E * l3 + q.A * l4;
 }
 
-// One Rayleigh layer: e <- C(layer) e, then rescale; returns the scale.
+// One Rayleigh layer: e <- C(layer) e, then rescale; returns the scale. The
+// same entries as rayleigh_compound, contracted as they are formed so that
+// few of them are live at once (the dual-number instantiations would spill).
 template <class T>
 MDR_HD T rayleigh_layer(const Layer<T> &ly, T k, T k2, T e[5]) {
-  const Compound<T> q = rayleigh_compound(ly, k, k2);
-  T n[5];
-  compound_apply(q, e, n);
-#pragma unroll
-  for (int i = 0; i < 5; ++i) e[i] = n[i];
+  const Wave<T> P = wave_factor(k, ly.ka, ly.h);
+  const Wave<T> S = wave_factor(k, ly.kb, ly.h);
+  const T a0 = P.att * S.att;
+  const T cpcq = P.c * S.c, cpy = P.c * S.w, cpz = P.c * S.x, cqw = S.c * P.w, cqx = S.c * P.x;
+  const T xy = P.x * S.w, xz = P.x * S.x, wy = P.w * S.w, wz = P.w * S.x;
+  const T gk = ly.gk, gam = gk * k2, gamm1 = gam - T(1), twgm1 = gam + gamm1;
+  const T gmgmk = gam * gk, gmgm1 = gam * gamm1, gm1sq = gamm1 * gamm1;
+  const T rho = ly.rho, irho = ly.irho, irho2 = ly.irho2;
+  const T a0pq = a0 - cpcq, k4 = k2 * k2;
+  const T A = cpcq - T(2) * gmgm1 * a0pq - gmgmk * xz - k2 * gm1sq * wy;
+  const T D = (k2 * cpy - cqx) * irho;
+  const T I = -(twgm1 * a0pq + gk * xz + k2 * gamm1 * wy) * irho;
+  const T C = (cpz - k2 * cqw) * irho;
+  const T Lq = -(T(2) * k2 * a0pq + xz + k4 * wy) * irho2;
+  const T E = (gmgmk * cpz - gm1sq * cqw) * rho;
+  const T B = cpcq;
+  const T G = gk * cpz - gamm1 * cqw;
+  const T M = -wz;
+  const T N = -xy;
+  const T F = (gm1sq * cpy - gmgmk * cqx) * rho;
+  const T H = gamm1 * cpy - gk * cqx;
+  const T O = -(T(2) * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * (rho * rho);
+  const T J = -(gk * gamm1 * twgm1 * a0pq + gam * gk * gk * xz + gamm1 * gm1sq * wy) * rho;
+  const T K = a0 + T(2) * (cpcq - A);
+  const T e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3], e4 = e[4];
+  const T e2t = e2 * (T(-2) * k2);
+  e[0] = m_fma(A, e0, m_fma(E, e1, m_fma(F, e3, m_fma(O, e4, J * e2t))));
+  e[1] = m_fma(D, e0, m_fma(B, e1, m_fma(N, e3, m_fma(F, e4, H * e2t))));
+  e[2] = m_fma(I, e0, m_fma(G, e1, m_fma(K, e2, m_fma(H, e3, J * e4))));
+  e[3] = m_fma(C, e0, m_fma(M, e1, m_fma(B, e3, m_fma(E, e4, G * e2t))));
+  e[4] = m_fma(Lq, e0, m_fma(C, e1, m_fma(D, e3, m_fma(A, e4, I * e2t))));
   return rescale<5>(e);
 }
 
