@@ -557,72 +557,158 @@ __global__ __launch_bounds__(kBlock) void unc_probe_kernel(PipeArgs a) {
 // FP64 probe of same-sign troughs: golden-section search for the minimum of
 // sign*F over [c_{i-1}, c_{i+1}], stopping at the first opposite-sign point.
 // ---------------------------------------------------------------------------
+// Golden-section search as a per-lane state machine: minimise sign*F
+// (trough) or the log2 scale (dip) over the probe region, stopping at the
+// first point where F takes the opposite sign.
+struct GoldenState {
+  double lo, hi, p, r, fp, fr, sign, pending;
+  float kp, kr, kmin, sc;
+  int it, stage, which;  // stage 0/1: first two points; 2: iterating. which: 0 p, 1 r
+  bool dip;
+};
+
+__device__ __forceinline__ void golden_init(GoldenState &s, double lo, double hi, double sign, bool dip, float sc) {
+  constexpr double phi = 0.6180339887498949;
+  s.lo = lo; s.hi = hi; s.sign = sign; s.dip = dip; s.sc = sc;
+  s.p = hi - phi * (hi - lo);
+  s.r = lo + phi * (hi - lo);
+  s.it = 0; s.stage = 0; s.which = 0;
+  s.fp = s.fr = 0.; s.kp = s.kr = 0.f;
+  s.kmin = INFINITY;
+}
+
+// 0: F is needed at s.pending; 1: finished with `split` (NaN: no pair found).
+__device__ __forceinline__ int golden_step(GoldenState &s, double &split, double &fsplit) {
+  constexpr double phi = 0.6180339887498949;
+  if (s.stage < 2) { s.which = s.stage; s.pending = s.stage ? s.r : s.p; return 0; }
+  if (s.it >= kTroughIters) { split = NAN; return 1; }
+  if (s.fp <= 0.) { split = s.p; fsplit = s.sign * s.fp; return 1; }
+  if (s.fr <= 0.) { split = s.r; fsplit = s.sign * s.fr; return 1; }
+  // A dip that does not deepen like log|c - c*| is not a collapsing resonance.
+  if (s.dip && s.it >= kDipStopIter && s.kmin > s.sc - kDipStopDrop) { split = NAN; return 1; }
+  if (s.kp < s.kr) {
+    s.hi = s.r; s.r = s.p; s.fr = s.fp; s.kr = s.kp;
+    s.p = s.hi - phi * (s.hi - s.lo);
+    s.which = 0; s.pending = s.p;
+  } else {
+    s.lo = s.p; s.p = s.r; s.fp = s.fr; s.kp = s.kr;
+    s.r = s.lo + phi * (s.hi - s.lo);
+    s.which = 1; s.pending = s.r;
+  }
+  ++s.it;
+  return 0;
+}
+
+__device__ __forceinline__ void golden_take(GoldenState &s, double x, double f, float key) {
+  if (s.which == 0) { s.p = x; s.fp = f; s.kp = key; } else { s.r = x; s.fr = f; s.kr = key; }
+  if (s.stage < 2) {
+    if (++s.stage == 2) s.kmin = fminf(s.kmin, fminf(s.kp, s.kr));
+  } else {
+    s.kmin = fminf(s.kmin, key);
+  }
+}
+
+// Event::probe layout: bits 0-7 the probe kind; bit 8 set when the FP32 pass
+// handed the search to the FP64 pass; bits 16-23 the iterations done by then.
+// While handed over, Event::x/vx hold the current bracket and Event::pad the
+// best key (float bits), until the FP64 pass writes the split.
+constexpr int kProbeNeeds64 = 0x100;
+
+// Probes of same-sign troughs and scale dips, one lane per probe with
+// lane-persistent scheduling. T = float: the FP32 pass of the FP32 path -- when
+// a point falls inside the FP32 uncertainty band (or the cutoff guard, or the
+// bracket has narrowed to the FP32 spacing of k) the search is handed, with its
+// bracket, iteration count and best key, to the FP64 pass. T = double: every
+// probe of the FP64 path, or the handed-over probes of the FP32 path, in FP64;
+// a resumed search restarts the golden section on the handed-over bracket (one
+// extra evaluation: the retained interior point sits at a golden point of it).
+template <class T>
 __global__ __launch_bounds__(kBlock) void trough_kernel(PipeArgs a) {
+  constexpr bool fp32 = sizeof(T) == 4;
   const int n = min(*a.trough_count, a.queue_cap);
-  for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
-    const uint32_t g = a.trough_queue[q];
-    const int tl = g / kEventCap;
-    Event &x = a.events[g];
-    if (x.probe != kTrough && x.probe != kDip) continue;
-    const bool dip = x.probe == kDip;
-    // Search only the neighbouring cells whose end values share the node's sign.
-    const NodeValues nv = node_values(a.events + static_cast<size_t>(tl) * kEventCap, g % kEventCap, a.scan[tl].nev);
-    const bool left = x.c0 == x.c0 && signbit(nv.l) == signbit(nv.c);
-    const bool right = x.c2 == x.c2 && signbit(nv.r) == signbit(nv.c);
-    if (!left && !right) continue;
-    const TraceInfo ti = a.traces[tl + a.trace_begin];
-    const ModelInfo mi = a.models[ti.model];
-    const PrepRow *rows = a.rows + mi.row0;
-    double lo = left ? x.c0 : x.c1, hi = right ? x.c2 : x.c1;
-    const double sign = signbit(nv.c) ? -1. : 1.;
-    // Golden-section minimization of sign*F (trough) or of the log2 scale
-    // (dip), stopping at the first point where F takes the opposite sign.
-    // FP32 path: probe points are evaluated in FP32 and rechecked in FP64 only
-    // inside the FP32 uncertainty band (or the cutoff guard), as in the scan.
-    const float tau = ti.tau;
-    const double iw2 = 1. / (ti.omega * ti.omega);
-    auto probe = [&](double &c, float &key) {
-      float ls;
-      // (FP32 only while the interval is far wider than the FP32 spacing of k.)
-      if (a.fp32 && c < mi.cutoff_guard && hi - lo > 1e-5 * hi) {
-        const double cs = snap32(ti.omega, c, lo, hi);
-        const float f32 = secular_global<float>(rows, mi.L, mi.top, a.wave, ti.omega, iw2, float(ti.omega / cs), &ls);
-        if (fabsf(f32) >= tau && cs > lo && cs < hi) {
-          c = cs;  // the point actually evaluated
-          const double f = sign * double(f32);
-          key = dip ? ls : float(f);
-          return f;
+  const int stride = gridDim.x * kBlock;
+  int q = blockIdx.x * kBlock + threadIdx.x - stride;
+  GoldenState s{};
+  bool active = false;
+  uint32_t g = 0;
+  const PrepRow *rows = nullptr;
+  int L = 0, top = 0;
+  double omega = 0., iw2 = 0., guard = 0.;
+  float tau = 0.f;
+  bool exhausted = false;
+  // Each pass: lanes fetch probes and take golden steps until they need one
+  // evaluation of F (or run out of work); then the wave evaluates together.
+  // The outer exit is wave-uniform.
+  for (;;) {
+    while (!exhausted) {
+      if (!active) {
+        q += stride;
+        if (q >= n) { exhausted = true; break; }
+        g = a.trough_queue[q];
+        const int tl = g / kEventCap;
+        const Event &x = a.events[g];
+        const int kind = x.probe & 0xff;
+        if (kind != kTrough && kind != kDip) continue;
+        const bool resumed = !fp32 && a.fp32;
+        if (resumed && !(x.probe & kProbeNeeds64)) continue;
+        // Search only the neighbouring cells whose end values share the node's sign.
+        const NodeValues nv = node_values(a.events + static_cast<size_t>(tl) * kEventCap, g % kEventCap,
+                                          a.scan[tl].nev);
+        const bool left = x.c0 == x.c0 && signbit(nv.l) == signbit(nv.c);
+        const bool right = x.c2 == x.c2 && signbit(nv.r) == signbit(nv.c);
+        if (!left && !right) continue;
+        const TraceInfo ti = a.traces[tl + a.trace_begin];
+        const ModelInfo mi = a.models[ti.model];
+        rows = a.rows + mi.row0;
+        L = mi.L;
+        top = mi.top;
+        omega = ti.omega;
+        iw2 = 1. / (omega * omega);
+        guard = mi.cutoff_guard;
+        tau = ti.tau;
+        const double sign = signbit(nv.c) ? -1. : 1.;
+        if (resumed) {
+          golden_init(s, x.x, x.vx, sign, kind == kDip, x.sc);
+          s.it = (x.probe >> 16) & 0xff;
+          s.kmin = __int_as_float(x.pad);
+        } else {
+          golden_init(s, left ? x.c0 : x.c1, right ? x.c2 : x.c1, sign, kind == kDip, x.sc);
+        }
+        active = true;
+      }
+      double split, fsplit = 0.;
+      if (!golden_step(s, split, fsplit)) break;  // needs F at s.pending
+      a.events[g].x = split;
+      a.events[g].vx = fsplit;
+      active = false;
+    }
+    if (!__ballot(active)) break;
+    if (!active) continue;
+    float ls;
+    if constexpr (fp32) {
+      // FP32 only while the interval is far wider than the FP32 spacing of k.
+      const double cs = snap32(omega, s.pending, s.lo, s.hi);
+      bool handed = !(cs < guard && s.hi - s.lo > 1e-5 * s.hi && cs > s.lo && cs < s.hi);
+      if (!handed) {
+        const float f = secular_global<float>(rows, L, top, a.wave, omega, iw2, float(omega / cs), &ls);
+        handed = !(fabsf(f) >= tau);
+        if (!handed) {
+          const double sf = s.sign * double(f);
+          golden_take(s, cs, sf, s.dip ? ls : float(sf));
         }
       }
-      const double f = sign * secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, c, &ls);
-      key = dip ? ls : float(f);
-      return f;
-    };
-    constexpr double phi = 0.6180339887498949;
-    double p = hi - phi * (hi - lo), r = lo + phi * (hi - lo);
-    float kp, kr;
-    double fp = probe(p, kp), fr = probe(r, kr);
-    float kmin = fminf(kp, kr);
-    double split = NAN, fsplit = 0.;
-    for (int it = 0; it < kTroughIters; ++it) {
-      if (fp <= 0.) { split = p; fsplit = sign * fp; break; }
-      if (fr <= 0.) { split = r; fsplit = sign * fr; break; }
-      // A dip that does not deepen like log|c - c*| is not a collapsing resonance.
-      if (dip && it >= kDipStopIter && kmin > x.sc - kDipStopDrop) break;
-      if (kp < kr) {
-        hi = r; r = p; fr = fp; kr = kp;
-        p = hi - phi * (hi - lo);
-        fp = probe(p, kp);
-        kmin = fminf(kmin, kp);
-      } else {
-        lo = p; p = r; fp = fr; kp = kr;
-        r = lo + phi * (hi - lo);
-        fr = probe(r, kr);
-        kmin = fminf(kmin, kr);
+      if (handed) {
+        Event &x = a.events[g];
+        x.x = s.lo;
+        x.vx = s.hi;
+        x.pad = __float_as_int(s.stage == 2 ? s.kmin : INFINITY);
+        x.probe |= kProbeNeeds64 | (s.it << 16);
+        active = false;
       }
+    } else {
+      const double f = s.sign * secular_rows(rows, L, top, a.wave, omega, s.pending, &ls);
+      golden_take(s, s.pending, f, s.dip ? ls : float(f));
     }
-    x.x = split;
-    x.vx = fsplit;
   }
 }
 
@@ -839,25 +925,100 @@ __device__ double brent(const PrepRow *rows, int L, int top, int wave, double om
   return NAN;
 }
 
+// Brent-Dekker as a per-lane state machine (the same steps and stop rule as
+// brent() above): a lane whose root converges takes its next bracket at
+// once, so every FP64 secular evaluation runs with nearly all lanes busy
+// instead of waiting for the wave's slowest bracket.
+struct BrentState {
+  double a, b, c, fa, fb, fc, d, e, stop;
+  int iter;
+};
+
+// Advance to the next evaluation point (returned through s.b) or finish:
+// returns 0 when F(s.b) is needed, 1 when `root` is final.
+__device__ __forceinline__ int brent_step(BrentState &s, double &root) {
+  if ((s.fb < 0.) == (s.fc < 0.)) { s.c = s.a; s.fc = s.fa; s.e = s.d = s.b - s.a; }
+  if (fabs(s.fc) < fabs(s.fb)) { s.a = s.b; s.fa = s.fb; s.b = s.c; s.fb = s.fc; s.c = s.a; s.fc = s.fa; }
+  if (s.fb == 0.) { root = s.b; return 1; }
+  const double mid = .5 * (s.c - s.b), step = .5 * s.stop;
+  if (fabs(s.c - s.b) <= s.stop) { root = .5 * (s.b + s.c); return 1; }
+  if (s.iter >= 160) { root = NAN; return 1; }
+  if (fabs(s.e) >= step && fabs(s.fa) > fabs(s.fb)) {
+    const double sr = s.fb / s.fa;
+    double pp, qq;
+    if (s.a == s.c) { pp = 2. * mid * sr; qq = 1. - sr; }
+    else {
+      const double q0 = s.fa / s.fc, r = s.fb / s.fc;
+      pp = sr * (2. * mid * q0 * (q0 - r) - (s.b - s.a) * (r - 1.));
+      qq = (q0 - 1.) * (r - 1.) * (sr - 1.);
+    }
+    if (pp > 0.) qq = -qq; else pp = -pp;
+    const double old = s.e;
+    s.e = s.d;
+    const double ratio = pp / qq;
+    if (ratio == ratio && fabs(ratio) < INFINITY && 2. * pp < fmin(3. * mid * qq - fabs(step * qq), fabs(old * qq)))
+      s.d = ratio;
+    else { s.d = mid; s.e = mid; }
+  } else { s.d = mid; s.e = mid; }
+  s.a = s.b;
+  s.fa = s.fb;
+  s.b += fabs(s.d) > step ? s.d : (mid > 0. ? step : -step);
+  ++s.iter;
+  return 0;
+}
+
 __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
   const int ntl = a.trace_end - a.trace_begin;
   const int total = ntl * a.kmax;
-  for (int q = blockIdx.x * kBlock + threadIdx.x; q < total; q += gridDim.x * kBlock) {
-    const int tl = q / a.kmax, m = q % a.kmax;
-    const TraceScan &ts = a.scan[tl];
-    const TraceInfo ti = a.traces[tl + a.trace_begin];
-    const ModelInfo mi = a.models[ti.model];
-    double root = NAN;
-    if (m < ts.count && m < mi.K) {
-      const Bracket br = a.brackets[q];
-      int ev = 0;
-      // Guard roots beyond the requested modes only feed the close-root test.
-      const double tol = m < mi.modes ? a.tol : fmax(a.tol, kGuardTol);
-      if (br.lo == br.lo) {
-        root = brent(a.rows + mi.row0, mi.L, mi.top, a.wave, ti.omega, br.lo, br.hi, br.flo, br.fhi, tol, ev);
+  const int stride = gridDim.x * kBlock;
+  int q = blockIdx.x * kBlock + threadIdx.x - stride;  // advanced before each fetch
+  BrentState s{};
+  bool active = false;
+  const PrepRow *rows = nullptr;
+  int L = 0, top = 0;
+  double omega = 0.;
+  bool exhausted = false;
+  // Each pass: lanes fetch brackets and take Brent steps until they need one
+  // evaluation of F (or run out of work); then the wave evaluates together.
+  // The outer exit is wave-uniform.
+  for (;;) {
+    while (!exhausted) {
+      if (!active) {
+        q += stride;
+        if (q >= total) { exhausted = true; break; }
+        const int tl = q / a.kmax, m = q % a.kmax;
+        const int count = a.scan[tl].count;
+        const TraceInfo ti = a.traces[tl + a.trace_begin];
+        const ModelInfo mi = a.models[ti.model];
+        const Bracket br = m < count && m < mi.K ? a.brackets[q] : Bracket{NAN, NAN, 0., 0.};
+        if (!(br.lo == br.lo)) { a.roots[q] = NAN; continue; }
+        if (br.flo == 0.) { a.roots[q] = br.lo; continue; }
+        if (br.fhi == 0.) { a.roots[q] = br.hi; continue; }
+        // Guard roots beyond the requested modes only feed the close-root test.
+        const double tol = m < mi.modes ? a.tol : fmax(a.tol, kGuardTol);
+        const double mag = fmax(fabs(br.lo), fabs(br.hi));
+        s.a = br.lo; s.b = br.hi; s.fa = br.flo; s.fb = br.fhi;
+        s.c = s.b; s.fc = s.fb; s.d = s.b - s.a; s.e = s.d;
+        s.stop = fmax(16. * 2.220446049250313e-16 * mag, fmin((s.b - s.a) / 64., .125 * tol * mag));
+        s.iter = 0;
+        rows = a.rows + mi.row0;
+        L = mi.L;
+        top = mi.top;
+        omega = ti.omega;
+        active = true;
       }
+      double root;
+      if (!brent_step(s, root)) break;  // needs F(b)
+      a.roots[q] = root;
+      active = false;
     }
-    a.roots[q] = root;
+    if (!__ballot(active)) break;
+    if (!active) continue;
+    s.fb = secular_rows(rows, L, top, a.wave, omega, s.b);
+    if (!(s.fb == s.fb) || fabs(s.fb) == INFINITY) {
+      a.roots[q] = NAN;
+      active = false;
+    }
   }
 }
 
@@ -1354,7 +1515,8 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
   hipLaunchKernelGGL(compact_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   hipLaunchKernelGGL(recheck_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   if (fp32) hipLaunchKernelGGL(unc_probe_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
-  hipLaunchKernelGGL(trough_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+  if (fp32) hipLaunchKernelGGL(trough_kernel<float>, dim3(pb), dim3(kBlock), 0, s, pa);
+  hipLaunchKernelGGL(trough_kernel<double>, dim3(pb), dim3(kBlock), 0, s, pa);
   mark(3);
   hipLaunchKernelGGL(assemble_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   if (pa.wave == 1) {
