@@ -175,6 +175,33 @@ int md_rocm_bounds(const double *thickness, const double *vp, const double *vs, 
 int md_rocm_probe_grid(const double *thickness, const double *vp, const double *vs, const double *rho, int B, int L,
                        const double *periods, int nf, const double *bounds, int nc, int wave, int fp32, double *out);
 
+/* Compiled resources of a search kernel (the fields of the reference's
+ * qe_kernel_report / love_phase_kernel_report dictionaries). "Per CU" here is
+ * the reference's "per SM". */
+typedef struct md_rocm_kernel_report {
+  int registers_per_thread;     /* VGPRs */
+  int local_bytes_per_thread;   /* scratch (register spills) */
+  int static_shared_bytes;      /* LDS */
+  int dynamic_shared_bytes;
+  int search_block_threads;
+  int max_active_blocks_per_cu; /* occupancy at that block size */
+  int configured_blocks_per_cu; /* persistent consumer blocks per CU (0 in the config = 8) */
+  int table_capacity_per_model; /* layer rows held in LDS by the scan */
+  int cache_points_per_trace;   /* scan events kept per trace */
+} md_rocm_kernel_report;
+
+/* The kernel that scans a model of L rows (1 <= L <= 256) with the FP32
+ * (single_precision = 1) or FP64 evaluator, at the given config's scan shape
+ * (cfg may be NULL). accuracy and shape select among the reference's CUDA
+ * search kernels; this backend has one scan kernel for all of them.
+ * Reference: qe_kernel_report (cpp/cuda/qe.cu:1143-1164). */
+int md_rocm_qe_kernel_report(int L, int accuracy, int shape, int single_precision, const md_rocm_config *cfg,
+                             md_rocm_kernel_report *out);
+
+/* The Love phase search kernel. Reference: love_phase_kernel_report
+ * (cpp/cuda/qe.cu:1131-1139). */
+int md_rocm_love_phase_kernel_report(int L, int single_precision, md_rocm_kernel_report *out);
+
 /* Device helpers (reference cpp/cuda/qe.cu:1140-1142, bindings.cpp:354-357). */
 int md_rocm_available(int device); /* 1 if the device exists and is gfx950-capable */
 int md_rocm_set_device(int device);

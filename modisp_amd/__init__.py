@@ -71,6 +71,38 @@ def device_name(device: int = 0) -> str:
     return buf.value.decode()
 
 
+def qe_kernel_report(L, accuracy=0, shape=False, single_precision=False, config=None):
+    """Compiled resources of the scan kernel that serves models of L rows
+    (reference qe_kernel_report, cpp/cuda/qe.cu:1143-1164: same keys; "per
+    SM" figures are per CU here). ``accuracy`` and ``shape`` pick among the
+    reference's CUDA search kernels; this backend has one scan kernel, FP32
+    (``single_precision``) or FP64. ``config`` is a solver's ``config`` (scan
+    shape), default the library's."""
+    r = _native.KernelReport()
+    cfg = ctypes.byref(config) if config is not None else None
+    _native.check(_native.lib().md_rocm_qe_kernel_report(int(L), int(accuracy), int(bool(shape)),
+                                                         int(bool(single_precision)), cfg, ctypes.byref(r)))
+    return dict(registers_per_thread=r.registers_per_thread, static_shared_bytes=r.static_shared_bytes,
+                local_bytes_per_thread=r.local_bytes_per_thread, search_block_threads=r.search_block_threads,
+                max_active_blocks_per_sm=r.max_active_blocks_per_cu,
+                configured_blocks_per_sm=r.configured_blocks_per_cu, dynamic_shared_bytes=r.dynamic_shared_bytes,
+                cache_points_per_trace=r.cache_points_per_trace,
+                table_capacity_per_model=r.table_capacity_per_model,
+                search_kernel="adaptive_scan_fp32" if single_precision else "adaptive_scan_fp64")
+
+
+def love_phase_kernel_report(L, single_precision=False):
+    """Compiled resources of the Love phase search kernel (reference
+    love_phase_kernel_report, cpp/cuda/qe.cu:1131-1139: same keys). The phase
+    search is FP64 for both scan precisions."""
+    r = _native.KernelReport()
+    _native.check(_native.lib().md_rocm_love_phase_kernel_report(int(L), int(bool(single_precision)),
+                                                                 ctypes.byref(r)))
+    return dict(registers_per_thread=r.registers_per_thread, local_bytes_per_thread=r.local_bytes_per_thread,
+                static_shared_bytes=r.static_shared_bytes, dynamic_shared_bytes=r.dynamic_shared_bytes,
+                search_block_threads=r.search_block_threads)
+
+
 def _rows(a, name):
     a = np.ascontiguousarray(a, dtype=np.float64)
     if a.ndim != 2:

@@ -27,6 +27,13 @@ class Config(ctypes.Structure):
                 ("graph", ctypes.c_int), ("scan_waves", ctypes.c_int), ("consumer_blocks", ctypes.c_int)]
 
 
+class KernelReport(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "registers_per_thread", "local_bytes_per_thread", "static_shared_bytes", "dynamic_shared_bytes",
+        "search_block_threads", "max_active_blocks_per_cu", "configured_blocks_per_cu", "table_capacity_per_model",
+        "cache_points_per_trace")]
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -40,7 +47,8 @@ EXPORTS = ("md_rocm_default_config", "md_rocm_create", "md_rocm_free", "md_rocm_
            "md_rocm_run_ragged", "md_rocm_synchronize", "md_rocm_last_timing", "md_rocm_bounds",
            "md_rocm_available", "md_rocm_set_device", "md_rocm_memory_info", "md_rocm_device_name",
            "md_rocm_last_error", "md_rocm_probe_grid", "md_rocm_last_stats", "md_rocm_adjoint",
-           "md_rocm_set_stream", "md_rocm_autotune")
+           "md_rocm_set_stream", "md_rocm_autotune", "md_rocm_qe_kernel_report",
+           "md_rocm_love_phase_kernel_report")
 
 
 def lib():
@@ -107,6 +115,11 @@ def lib():
     L.md_rocm_probe_grid.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
     L.md_rocm_probe_grid.restype = ctypes.c_int
+    L.md_rocm_qe_kernel_report.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(Config), ctypes.POINTER(KernelReport)]
+    L.md_rocm_qe_kernel_report.restype = ctypes.c_int
+    L.md_rocm_love_phase_kernel_report.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(KernelReport)]
+    L.md_rocm_love_phase_kernel_report.restype = ctypes.c_int
     L.md_rocm_last_error.argtypes = []
     L.md_rocm_last_error.restype = ctypes.c_char_p
     _lib = L

@@ -13,10 +13,11 @@ selection falls through to its CPU backend.
 The engine is this package's ``QeSolver`` (FP32 density-adaptive grid scan,
 FP64 recheck, FP64 Brent refinement, analytic FP64 group velocity) for every
 ``accuracy`` setting; ``accuracy`` selects the default refinement tolerance
-as in the reference. ``frequency_strategy="anchor_continuation"`` is accepted
-and reported as falling back to independent search (the plugin advertises
-``SUPPORTS_CONTINUATION = False``), as the reference does for an engine
-without continuation.
+as in the reference. ``frequency_strategy="anchor_continuation"`` runs the
+engine's anchor continuation under the reference's conditions (not at
+``accuracy="high"``, at least 8 unique periods, an interior target within
+the anchor log-gap limit; python/modisp/__init__.py:324-341) on the sorted
+unique periods, and reports prediction flags.
 """
 from __future__ import annotations
 
@@ -263,6 +264,7 @@ class Solver:
         self._calls = 0
         self._last_stats = None
         self.last_result = None
+        self._key = None
         self._lock = threading.RLock()
 
     # -- backend selection (reference __init__.py:255-271) --------------------------------------
@@ -319,26 +321,47 @@ class Solver:
             raise ValueError("periods must be a finite positive 1-D array")
         backend, reason = self._select()
         input_periods = per
+        unique_periods = per
         inverse = None
-        # Unsorted or repeated periods are solved once each (reference 314-341, restored at 463-475).
-        if per.size > 1 and not np.all(per[1:] > per[:-1]) and not np.all(per[1:] < per[:-1]):
-            per, inverse = np.unique(per, return_inverse=True)
+        # Unsorted or repeated periods are solved once each (reference 314-341, restored at 463-475);
+        # continuation needs them sorted.
+        if per.size > 1 and not np.all(per[1:] > per[:-1]):
+            needs_sorted = self.frequency_strategy == "anchor_continuation" and self.accuracy != "high" \
+                and per.size >= 8
+            if needs_sorted or not np.all(per[1:] < per[:-1]):
+                unique_periods, inverse = np.unique(per, return_inverse=True)
+        continuation = False
         frequency_reason = "independent frequency search requested"
         if self.frequency_strategy == "anchor_continuation":
-            frequency_reason = "ROCm plugin lacks continuation support; using independent search"
+            if self.accuracy == "high":
+                frequency_reason = "high accuracy retains independent search"
+            elif unique_periods.size < 8:
+                frequency_reason = "fewer than 8 unique periods; using independent search"
+            elif not np.any(np.log(unique_periods[2:]) - np.log(unique_periods[:-2]) <= .25):
+                frequency_reason = ("period grid has no interior target within the anchor log-gap limit; "
+                                    "using independent search")
+            else:
+                continuation = True
+                frequency_reason = "ROCM anchor continuation on sorted unique periods"
+        if continuation or unique_periods.size < per.size:
+            per = unique_periods
+        else:
+            inverse = None  # keep the caller's order for independent, all-unique grids
+        frequency_used = "anchor_continuation" if continuation else "independent"
         wave_code = int(wave == "love")
         bnd = _bounds(h, a, b, r, wave_code, self.cmin, self.cmax)
         if not np.isfinite(bnd).all() or np.any(bnd[:, 0] <= 0):
             raise ValueError("automatic search range could not be constructed")
         capacity = min(B, self.chunk_size or B)
         key = (capacity, L, per.size, int(nmodes), self.precision, self.group_velocity, self.group_rtol,
-               self.close_root_rtol)
+               self.close_root_rtol, continuation, self.anchor_stride)
         allocated = self._engine_key is None or self._engine_key[1:] != key[1:] or self._engine_key[0] < capacity
         if allocated:
             self.close()
             self._engine = QeSolver(capacity, L, per.size, int(nmodes), close_root_rtol=self.close_root_rtol,
                                     single_precision=self.precision == "float32",
-                                    group_velocity=self.group_velocity, group_rtol=self.group_rtol)
+                                    group_velocity=self.group_velocity, group_rtol=self.group_rtol,
+                                    continuation=continuation, anchor_stride=self.anchor_stride)
             self._engine_key = key
         else:
             capacity = self._engine_key[0]
@@ -348,6 +371,8 @@ class Solver:
         counts = np.empty_like(levels)
         group = np.empty(velocity.shape, dtype=np.float64) if self.group_velocity else None
         gflags = np.empty(velocity.shape, dtype=np.uint32) if self.group_velocity else None
+        prediction = (np.full((B, per.size), int(PredictionFlag.BYPASSED), dtype=np.uint32)
+                      if self.frequency_strategy == "anchor_continuation" else None)
         stats = {k: 0 for k in STAT_NAMES}
         chunks = 0
         for start in range(0, B, capacity):
@@ -358,6 +383,17 @@ class Solver:
             if self.group_velocity:
                 group[start:end] = extras["group_velocity"]
                 gflags[start:end] = extras["group_velocity_flags"]
+            if continuation:
+                chunk_prediction = extras["prediction_flags"]
+                prediction[start:end] = chunk_prediction
+                for name, bit in (("root_count_unsafe", 128), ("anchor_unsafe", 256), ("window_unsafe", 512),
+                                  ("bracket_failed", 1024), ("correction_failed", 2048), ("audit_failed", 32)):
+                    counter = "continuation_fallback_" + name
+                    stats[counter] = stats.get(counter, 0) + int(np.count_nonzero(chunk_prediction & bit))
+                stats["continuation_accepted"] = stats.get("continuation_accepted", 0) + int(
+                    np.count_nonzero(chunk_prediction & int(PredictionFlag.ACCEPTED)))
+                stats["continuation_fallbacks"] = stats.get("continuation_fallbacks", 0) + int(
+                    np.count_nonzero(chunk_prediction & int(PredictionFlag.FALLBACK)))
             for k, v in part.items():
                 stats[k] = stats.get(k, 0) + int(v)
             chunks += 1
@@ -366,6 +402,8 @@ class Solver:
             velocity, flags, levels, counts = (x.take(inverse, axis=-1) for x in (velocity, flags, levels, counts))
             if group is not None:
                 group, gflags = group.take(inverse, axis=-1), gflags.take(inverse, axis=-1)
+            if prediction is not None:
+                prediction = prediction.take(inverse, axis=-1)
         if self.cmin is not None or self.cmax is not None:
             flags |= TRACE_RANGE_TRUNCATED
         counts[(flags & TRACE_SEARCH_UNCONVERGED) != 0] = -1
@@ -375,10 +413,13 @@ class Solver:
                      close_root_rtol=self.close_root_rtol, refinement_tolerance=self.c_tol_rel, chunks=chunks,
                      capacity_models=int(capacity), engine_allocations=int(allocated),
                      group_velocity=self.group_velocity, group_rtol=self.group_rtol,
-                     frequency_strategy=self.frequency_strategy, frequency_strategy_used="independent",
+                     frequency_strategy=self.frequency_strategy, frequency_strategy_used=frequency_used,
                      frequency_strategy_reason=frequency_reason, anchor_stride=self.anchor_stride,
                      periods_requested=int(input_periods.size), periods_computed=int(computed_periods),
                      close_root_risk_traces=int(np.count_nonzero(flags & TRACE_CLOSE_ROOT_RISK)))
+        attempts = stats.get("continuation_accepted", 0) + stats.get("continuation_fallbacks", 0)
+        stats["continuation_local_success_rate"] = (stats.get("continuation_accepted", 0) / attempts
+                                                    if attempts else None)
         if self.profile:
             stats["timings_s"] = {"solver_total": time.perf_counter() - started}
         self._last_stats = stats
@@ -386,15 +427,46 @@ class Solver:
         result = DispersionBatch(velocity, input_periods, nmodes, wave, flags, levels, counts, bnd, backend, reason,
                                  accuracy=self.accuracy, close_root_rtol=self.close_root_rtol,
                                  group_velocity=group, group_velocity_flags=gflags,
+                                 prediction_flags=prediction,
                                  group_rtol=self.group_rtol if self.group_velocity else None,
-                                 frequency_strategy=self.frequency_strategy,
+                                 frequency_strategy=self.frequency_strategy, frequency_strategy_used=frequency_used,
                                  frequency_strategy_reason=frequency_reason, anchor_stride=self.anchor_stride)
+        self._key = (B, L, per.size, self.nc, int(nmodes))
         self.last_result = result
         return result
 
     @property
     def calls(self):
         return self._calls
+
+    def kernel_report(self):
+        """Settings of the last solve (or of this solver) and the compiled
+        resources of the scan kernel for its model depth, plus the Love phase
+        kernel after a Love solve (reference Solver.kernel_report,
+        python/modisp/__init__.py:517-557)."""
+        from . import qe_kernel_report, love_phase_kernel_report
+        last = self.last_result
+        precision = last.precision if last is not None else self.precision
+        report = dict(backend="rocm", precision=precision, accuracy=self.accuracy, secular=self.secular,
+                      secular_kernel=("sh" if last is not None and last.wave == "love" else "dunkin"),
+                      close_root_rtol=self.close_root_rtol, refinement_tolerance=self.c_tol_rel,
+                      refinement=self.refinement, love_strategy=self.love_strategy,
+                      group_velocity=self.group_velocity, group_rtol=self.group_rtol,
+                      frequency_strategy=self.frequency_strategy, anchor_stride=self.anchor_stride)
+        if last is not None:
+            report.update(frequency_strategy_used=last.frequency_strategy_used,
+                          frequency_strategy_reason=last.frequency_strategy_reason)
+        if self._key is not None:
+            cfg = self._engine.config if self._engine is not None else None
+            # The engine scans in FP32 for both output precisions.
+            report.update(qe_kernel_report(self._key[1], int(self.accuracy != "high"),
+                                           self.search_strategy == "shape_adaptive", True, config=cfg))
+            if last is not None and last.wave == "love":
+                report["scan_kernel_report"] = dict(report)
+                report.update(love_phase_kernel_report(self._key[1], precision == "float32"))
+                report["scan_kernel"] = report.get("search_kernel")
+                report["search_kernel"] = "love_phase_with_scan_fallback"
+        return report
 
     def stats(self):
         if self._last_stats is None:

@@ -1474,6 +1474,39 @@ static hipError_t launch_scan_t(const PipeArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <class K>
+static hipError_t report_of(K kernel, int block, KernelReport *r) {
+  hipFuncAttributes at{};
+  hipError_t e = hipFuncGetAttributes(&at, reinterpret_cast<const void *>(kernel));
+  if (e != hipSuccess) return e;
+  r->registers = at.numRegs;
+  r->local_bytes = static_cast<int>(at.localSizeBytes);
+  r->static_shared = static_cast<int>(at.sharedSizeBytes);
+  r->dynamic_shared = 0;
+  r->block = block;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&r->active_per_cu, kernel, block, 0);
+}
+
+template <class T>
+static hipError_t scan_report(int L, int waves, KernelReport *r) {
+  r->table_rows = L > kAdaptRows ? kMaxRows : kAdaptRows;
+  if (L > kAdaptRows) return report_of(scan_kernel<T, kMaxRows, 1>, 64, r);
+  if (waves == 1) return report_of(scan_kernel<T, kAdaptRows, 1>, 64, r);
+  if (waves == 2) return report_of(scan_kernel<T, kAdaptRows, 2>, 128, r);
+  if constexpr (sizeof(T) == 4)
+    if (waves == 8) return report_of(scan_kernel<T, kAdaptRows, 8>, 512, r);
+  return report_of(scan_kernel<T, kAdaptRows, 4>, 256, r);
+}
+
+hipError_t kernel_report(int which, int L, bool fp32, int scan_waves, KernelReport *r) {
+  r->events_per_trace = kEventCap;
+  if (which == 1) {
+    r->table_rows = L;
+    return report_of(love_phase_kernel, kBlock, r);
+  }
+  return fp32 ? scan_report<float>(L, scan_waves, r) : scan_report<double>(L, scan_waves, r);
+}
+
 hipError_t launch_continuation(const ContArgs &c, int persistent_blocks, hipStream_t s) {
   if (c.nanchors > 0) {
     hipLaunchKernelGGL(slope_kernel, dim3(persistent_blocks), dim3(kBlock), 0, s, c);

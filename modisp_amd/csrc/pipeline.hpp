@@ -90,8 +90,9 @@ struct Event {
   float sl, sc, sr;   // summed log2 normalization scales at c0, c1, c2
   int node;           // node index j within the trace
   int kind;           // EventKind of the scan classification
-  int probe;          // kTrough / kDip when the node is probed for a hidden pair
-  int pad;
+  int probe;          // bits 0-7: kTrough / kDip when the node is probed for a hidden pair;
+                      // bit 8 / bits 16-23: probe handed from FP32 to FP64, after that many steps
+  int pad;            // best key of a handed-over probe (float bits)
 };
 
 struct TraceScan {
@@ -296,6 +297,12 @@ struct AdjointArgs {
   int B, L, T, modes, row_periods, wave, tape_lanes;
 };
 hipError_t launch_adjoint(const AdjointArgs &a, int blocks, hipStream_t s);
+// Compiled resources of the kernel that scans a model of L rows (which = 0)
+// or of the Love phase search (which = 1), for the plugin's kernel reports.
+struct KernelReport {
+  int registers, local_bytes, static_shared, dynamic_shared, block, active_per_cu, table_rows, events_per_trace;
+};
+hipError_t kernel_report(int which, int L, bool fp32, int scan_waves, KernelReport *r);
 hipError_t launch_probe(const TraceInfo *traces, const ModelInfo *models, const PrepRow *rows, int ntraces, int nc,
                         int wave, int fp32, double *out, hipStream_t s);
 }  // namespace mdr

@@ -340,6 +340,35 @@ int md_rocm_device_name(int device, char *buf, int len) {
   return 0;
 }
 
+static int fill_report(int which, int L, int single_precision, const md_rocm_config *cfg, md_rocm_kernel_report *out) {
+  if (L < 1 || L > mdr::kMaxRows) return fail(2, "L must be 1..256");
+  if (!out) return fail(2, "null report");
+  mdr::KernelReport r{};
+  const int waves = cfg && cfg->scan_waves > 0 ? cfg->scan_waves : 4;
+  MD_HIP(mdr::kernel_report(which, L, single_precision != 0, waves, &r));
+  out->registers_per_thread = r.registers;
+  out->local_bytes_per_thread = r.local_bytes;
+  out->static_shared_bytes = r.static_shared;
+  out->dynamic_shared_bytes = r.dynamic_shared;
+  out->search_block_threads = r.block;
+  out->max_active_blocks_per_cu = r.active_per_cu;
+  out->configured_blocks_per_cu = cfg && cfg->consumer_blocks > 0 ? cfg->consumer_blocks : 8;
+  out->table_capacity_per_model = r.table_rows;
+  out->cache_points_per_trace = r.events_per_trace;
+  return 0;
+}
+
+int md_rocm_qe_kernel_report(int L, int accuracy, int shape, int single_precision, const md_rocm_config *cfg,
+                             md_rocm_kernel_report *out) {
+  (void)accuracy;
+  (void)shape;
+  return fill_report(0, L, single_precision, cfg, out);
+}
+
+int md_rocm_love_phase_kernel_report(int L, int single_precision, md_rocm_kernel_report *out) {
+  return fill_report(1, L, single_precision, nullptr, out);
+}
+
 md_rocm_solver *md_rocm_create(int capacity, int L, int nf, int nmodes, const md_rocm_config *cfg) {
   md_rocm_config c;
   md_rocm_default_config(&c);

@@ -35,8 +35,18 @@ def test_plugin_attributes_match_reference_loader_contract():
     for attr in ("SUPPORTS_FP32", "SUPPORTS_AUTO_ACCURACY", "SUPPORTS_CANONICAL_LOWER", "SUPPORTS_ROW_PERIODS",
                  "SUPPORTS_GROUP_VELOCITY", "SUPPORTS_CONTINUATION", "STAT_NAMES"):
         assert hasattr(module, attr)
-    for fn in ("available", "set_device", "memory_info", "device_name", "QeSolver"):
+    for fn in ("available", "set_device", "memory_info", "device_name", "QeSolver", "qe_kernel_report",
+               "love_phase_kernel_report"):
         assert callable(getattr(module, fn))
+
+
+def test_kernel_reports_validate_the_row_count_on_the_host():
+    # reference qe.cu:1132: "L must be 1..256"
+    for bad in (0, 257):
+        with pytest.raises(ValueError, match="L must be 1..256"):
+            modisp_amd.qe_kernel_report(bad)
+        with pytest.raises(ValueError, match="L must be 1..256"):
+            modisp_amd.love_phase_kernel_report(bad)
 
 
 def test_invalid_configuration_is_a_value_error_before_any_device_call():

@@ -129,3 +129,57 @@ def test_float32_precision_and_single_model_dispersion(gpu):
     assert r32.phase_velocity.dtype == np.float32
     np.testing.assert_array_equal(r32.mode_mask, r64.mode_mask)
     np.testing.assert_allclose(r32.phase_velocity[r32.mode_mask], r64.phase_velocity[r64.mode_mask], rtol=1e-6)
+
+
+@pytest.mark.gpu
+def test_anchor_continuation_front_end_equals_independent_search(gpu):
+    """frequency_strategy="anchor_continuation" follows the reference's
+    conditions (python/modisp/__init__.py:324-341): it runs on sorted unique
+    periods, reports prediction flags in the caller's period order, and every
+    root equals the independent search's."""
+    wl = workloads.precision_batch(B=16, seed=7)
+    per = np.geomspace(1., 40., 40)   # log step 0.095: interior targets within the 0.25 gap limit
+    shuffled = per[np.random.default_rng(3).permutation(per.size)]
+    common = dict(accuracy="balanced", c_tol_rel=1e-12, group_velocity=True)
+    with api.Solver(**common) as s:
+        ind = s(wl.thickness, wl.vp, wl.vs, wl.rho, shuffled, 4)
+    with api.Solver(frequency_strategy="anchor_continuation", **common) as s:
+        con = s(wl.thickness, wl.vp, wl.vs, wl.rho, shuffled, 4)
+        stats = s.stats()
+    assert ind.frequency_strategy_used == "independent" and ind.prediction_flags is None
+    assert con.frequency_strategy_used == "anchor_continuation", con.frequency_strategy_reason
+    assert con.frequency_strategy_reason == "ROCM anchor continuation on sorted unique periods"
+    np.testing.assert_array_equal(con.period, shuffled)
+    pred = con.prediction_flags
+    assert pred.shape == (16, per.size)
+    kinds = sum(((pred & int(bit)) != 0).astype(int) for bit in
+                (api.PredictionFlag.ANCHOR, api.PredictionFlag.ACCEPTED, api.PredictionFlag.FALLBACK))
+    assert np.all(kinds == 1)
+    assert stats["continuation_accepted"] == int(np.count_nonzero(pred & int(api.PredictionFlag.ACCEPTED)))
+    np.testing.assert_array_equal(con.mode_mask, ind.mode_mask)
+    ok = ind.mode_mask
+    assert np.max(np.abs(con.phase_velocity[ok] - ind.phase_velocity[ok]) / ind.phase_velocity[ok]) <= 1e-9
+    with api.Solver(frequency_strategy="anchor_continuation", accuracy="high", c_tol_rel=1e-12) as s:
+        high = s(wl.thickness, wl.vp, wl.vs, wl.rho, per, 4)
+    assert high.frequency_strategy_used == "independent"
+    assert high.frequency_strategy_reason == "high accuracy retains independent search"
+    assert np.all(high.prediction_flags == int(api.PredictionFlag.BYPASSED))
+
+
+@pytest.mark.gpu
+def test_kernel_report_has_the_reference_keys(gpu):
+    wl = workloads.plain_batch(B=2, seed=8)
+    with api.Solver(accuracy="high") as s:
+        s(wl.thickness, wl.vp, wl.vs, wl.rho, np.array([2., 5.]), 2, "love")
+        rep = s.kernel_report()
+    for key in ("registers_per_thread", "static_shared_bytes", "local_bytes_per_thread", "search_block_threads",
+                "max_active_blocks_per_sm", "configured_blocks_per_sm", "dynamic_shared_bytes",
+                "cache_points_per_trace", "table_capacity_per_model", "scan_kernel_report"):
+        assert key in rep, key
+    assert rep["search_kernel"] == "love_phase_with_scan_fallback"
+    scan = rep["scan_kernel_report"]
+    assert scan["local_bytes_per_thread"] == 0 and scan["max_active_blocks_per_sm"] >= 1
+    deep = gpu.qe_kernel_report(200, single_precision=True)
+    assert deep["table_capacity_per_model"] == 256 and deep["search_block_threads"] == 64
+    with pytest.raises(ValueError):
+        gpu.qe_kernel_report(0)
