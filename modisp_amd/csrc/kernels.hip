@@ -688,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void trough_kernel(PipeArgs a) {
     if constexpr (fp32) {
       // FP32 only while the interval is far wider than the FP32 spacing of k.
       const double cs = snap32(omega, s.pending, s.lo, s.hi);
-      bool handed = !(cs < guard && s.hi - s.lo > 1e-5 * s.hi && cs > s.lo && cs < s.hi);
+      bool handed = !(cs < guard && s.hi - s.lo > 2e-6 * s.hi && cs > s.lo && cs < s.hi);
       if (!handed) {
         const float f = secular_global<float>(rows, L, top, a.wave, omega, iw2, float(omega / cs), &ls);
         handed = !(fabsf(f) >= tau);
@@ -1108,9 +1108,13 @@ __device__ __forceinline__ double group_at(const PrepRow *rows, int L, int top, 
 
 __global__ __launch_bounds__(kBlock) void group_kernel(PipeArgs a) {
   const int ntl = a.trace_end - a.trace_begin;
-  const int total = ntl * a.kmax;
+  // Mode-major: neighbouring lanes take one mode of neighbouring traces (the
+  // next periods of one model), whose roots sit in the same layer regimes, so
+  // the propagating/evanescent branches of a wave mostly agree. Guard roots
+  // (the last two of kmax) have no group velocity and get no lane.
+  const int total = ntl * (a.kmax - 2);
   for (int q = blockIdx.x * kBlock + threadIdx.x; q < total; q += gridDim.x * kBlock) {
-    const int tl = q / a.kmax, m = q % a.kmax;
+    const int m = q / ntl, tl = q - m * ntl;
     const TraceInfo ti = a.traces[tl + a.trace_begin];
     const ModelInfo mi = a.models[ti.model];
     if (m >= mi.modes) continue;
