@@ -32,7 +32,8 @@ SUPPORTS_CANONICAL_LOWER = True
 SUPPORTS_ROW_PERIODS = True
 SUPPORTS_GROUP_VELOCITY = True
 SUPPORTS_CONTINUATION = True
-STAT_NAMES = ("traces", "fp64_rechecks", "pair_probes", "fp64_rescans", "love_phase_searches", "scan_nodes")
+STAT_NAMES = ("traces", "fp64_rechecks", "pair_probes", "fp64_rescans", "love_phase_searches", "scan_nodes",
+              "refine_evaluations", "probe_evaluations")
 
 __all__ = ["load", "QeSolver", "available", "set_device", "memory_info", "device_name", "bounds",
            "BackendUnavailableError", "ABI_VERSION", "VERSION"]

@@ -104,6 +104,13 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// Adds a per-lane count to a run statistic with one atomic per wavefront
+// (every lane of the wave must call it).
+__device__ __forceinline__ void wave_count(unsigned long long *stat, unsigned n) {
+  unsigned long long v = n;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(stat, v);
+}
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
 __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 #pragma unroll
@@ -623,7 +630,7 @@ constexpr int kProbeNeeds64 = 0x100;
 // a resumed search restarts the golden section on the handed-over bracket (one
 // extra evaluation: the retained interior point sits at a golden point of it).
 template <class T>
-__global__ __launch_bounds__(kBlock) void trough_kernel(PipeArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void trough_kernel(PipeArgs a) {
   constexpr bool fp32 = sizeof(T) == 4;
   const int n = min(*a.trough_count, a.queue_cap);
   const int stride = gridDim.x * kBlock;
@@ -635,6 +642,7 @@ __global__ __launch_bounds__(kBlock) void trough_kernel(PipeArgs a) {
   int L = 0, top = 0;
   double omega = 0., iw2 = 0., guard = 0.;
   float tau = 0.f;
+  unsigned evals = 0;
   bool exhausted = false;
   // Each pass: lanes fetch probes and take golden steps until they need one
   // evaluation of F (or run out of work); then the wave evaluates together.
@@ -684,6 +692,7 @@ __global__ __launch_bounds__(kBlock) void trough_kernel(PipeArgs a) {
     }
     if (!__ballot(active)) break;
     if (!active) continue;
+    ++evals;
     float ls;
     if constexpr (fp32) {
       // FP32 only while the interval is far wider than the FP32 spacing of k.
@@ -710,6 +719,7 @@ __global__ __launch_bounds__(kBlock) void trough_kernel(PipeArgs a) {
       golden_take(s, s.pending, f, s.dip ? ls : float(f));
     }
   }
+  if (a.stats) wave_count(a.stats + 7, evals);
 }
 
 // ---------------------------------------------------------------------------
@@ -974,6 +984,7 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
   int q = blockIdx.x * kBlock + threadIdx.x - stride;  // advanced before each fetch
   BrentState s{};
   bool active = false;
+  unsigned evals = 0;
   const PrepRow *rows = nullptr;
   int L = 0, top = 0;
   double omega = 0.;
@@ -1014,12 +1025,14 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
     }
     if (!__ballot(active)) break;
     if (!active) continue;
+    ++evals;
     s.fb = secular_rows(rows, L, top, a.wave, omega, s.b);
     if (!(s.fb == s.fb) || fabs(s.fb) == INFINITY) {
       a.roots[q] = NAN;
       active = false;
     }
   }
+  if (a.stats) wave_count(a.stats + 6, evals);
 }
 
 // ---------------------------------------------------------------------------
@@ -1419,7 +1432,9 @@ __global__ void reset_stats_kernel(unsigned long long *stats, int n) {
 
 // Accumulate this chunk's work counters into the run statistics:
 // [0] traces, [1] FP64 rechecks, [2] pair probes, [3] FP64 rescans,
-// [4] Love phase searches, [5] evaluated scan nodes.
+// [4] Love phase searches, [5] evaluated scan nodes (nodes_kernel),
+// [6] FP64 refinement evaluations, [7] probe evaluations (the consumer
+// kernels add these two themselves).
 __global__ void stats_kernel(const int *counters, unsigned long long *stats, int traces) {
   if (threadIdx.x == 0) {
     stats[0] += static_cast<unsigned long long>(traces);
@@ -1536,7 +1551,7 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
   // Run statistics accumulate over chunks; cleared by a kernel (not a memset
   // node) so that the sequence stays capturable.
   if (pa.stats && pa.trace_begin == 0)
-    hipLaunchKernelGGL(reset_stats_kernel, dim3(1), dim3(64), 0, s, pa.stats, 6);
+    hipLaunchKernelGGL(reset_stats_kernel, dim3(1), dim3(64), 0, s, pa.stats, kStats);
   mark(0);
   hipError_t err = fp32 ? launch_scan_t<float>(sa, s) : launch_scan_t<double>(sa, s);
   if (err != hipSuccess) return err;

@@ -19,6 +19,7 @@ namespace mdr {
 constexpr int kBlock = 256;       // lanes per scan block (4 wave64)
 constexpr int kMaxRows = 256;     // layer rows per model (reference limit, __init__.py:152)
 constexpr int kEventCap = 32;     // ascending events kept per trace
+constexpr int kStats = 8;         // run statistics (stats_kernel)
 constexpr int kFloor = 64;        // uniform table points of the adaptive grid (one per lane)
 constexpr int kAdaptRows = 64;    // rows of the common scan scratch; deeper models use a kMaxRows scratch
 constexpr int kCellMax = 1 << 16; // density nodes per table cell (cap)
@@ -144,7 +145,7 @@ struct PipeArgs {
   int32_t *counts;
   double *group;
   uint32_t *group_flags;
-  unsigned long long *stats;  // optional run statistics (see stats_kernel)
+  unsigned long long *stats;  // optional run statistics [kStats] (see stats_kernel)
   double *all_roots;  // optional: every (up to kmax) root per listed trace, [trace list position][kmax]
 };
 

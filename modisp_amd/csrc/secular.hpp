@@ -55,9 +55,43 @@ MDR_HD void m_sincos(float x, float *s, float *c) {
   *c = std::cos(x);
 #endif
 }
+// sin and cos in FP64 for the layer phases (|x| = k h, well below 2^20):
+// Cody-Waite reduction by pi/2 with a two-term split of pi/2 and FMAs, then
+// the Taylor series of sin (to x^17) and cos (to x^16) on [-pi/4, pi/4]
+// (truncation below 1e-18) and the quadrant map. Absolute error of a few ulp;
+// no large-argument path (the library routine carries one, and its branch).
+MDR_HD void sincos_phase(double x, double *s, double *c) {
+  const double n = std::rint(x * 0.6366197723675814);       // x / (pi/2)
+  double r = std::fma(-n, 1.5707963267948966, x);            // pi/2, leading double
+  r = std::fma(-n, 6.123233995736766e-17, r);                // pi/2 - leading double
+  const double z = r * r;
+  double ps = 2.8114572543455206e-15;                        // 1/17!
+  ps = std::fma(ps, z, -7.647163731819816e-13);              // -1/15!
+  ps = std::fma(ps, z, 1.6059043836821613e-10);              // 1/13!
+  ps = std::fma(ps, z, -2.505210838544172e-08);              // -1/11!
+  ps = std::fma(ps, z, 2.7557319223985893e-06);              // 1/9!
+  ps = std::fma(ps, z, -1.984126984126984e-04);              // -1/7!
+  ps = std::fma(ps, z, 8.333333333333333e-03);               // 1/5!
+  ps = std::fma(ps, z, -1.6666666666666666e-01);             // -1/3!
+  const double sn = std::fma(r * z, ps, r);
+  double pc = 4.779477332387385e-14;                         // 1/16!
+  pc = std::fma(pc, z, -1.1470745597729725e-11);             // -1/14!
+  pc = std::fma(pc, z, 2.08767569878681e-09);                // 1/12!
+  pc = std::fma(pc, z, -2.755731922398589e-07);              // -1/10!
+  pc = std::fma(pc, z, 2.48015873015873e-05);                // 1/8!
+  pc = std::fma(pc, z, -1.388888888888889e-03);              // -1/6!
+  pc = std::fma(pc, z, 4.1666666666666664e-02);              // 1/4!
+  pc = std::fma(pc, z, -0.5);                                // -1/2!
+  const double cs = std::fma(z, pc, 1.);
+  const int q = static_cast<int>(n) & 3;
+  const double a = (q & 1) ? cs : sn, b = (q & 1) ? sn : cs;
+  *s = (q & 2) ? -a : a;
+  *c = ((q + 1) & 2) ? -b : b;
+}
+
 MDR_HD void m_sincos(double x, double *s, double *c) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  sincos(x, s, c);
+  sincos_phase(x, s, c);
 #else
   *s = std::sin(x);
   *c = std::cos(x);
@@ -155,9 +189,23 @@ MDR_HD void root_pair(float r2, float *r, float *ir) {
   *ir = 1.f / *r;
 #endif
 }
+// FP64: 1/sqrt from an estimate by two Newton steps y <- y (3/2 - x y^2 / 2)
+// (each doubles the correct bits: the hardware estimate has 24, so the result
+// is within 2 ulp), and sqrt = x / sqrt(x).
+MDR_HD double rsq_newton(double x, double y) {
+  const double h = 0.5 * x;
+  y *= std::fma(-h * y, y, 1.5);
+  return y * std::fma(-h * y, y, 1.5);
+}
 MDR_HD void root_pair(double r2, double *r, double *ir) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = rsq_newton(r2, __builtin_amdgcn_rsq(r2));
+  *ir = y;
+  *r = r2 * y;
+#else
   *r = sqrt(r2);
   *ir = 1. / *r;
+#endif
 }
 template <class T>
 MDR_HD void root_pair(T r2, T *r, T *ir) {

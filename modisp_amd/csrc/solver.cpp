@@ -59,7 +59,7 @@ struct DevBuf {
   T *as() const { return static_cast<T *>(p); }
 };
 
-constexpr int kStatCount = 6;  // traces, FP64 rechecks, pair probes, FP64 rescans, Love phase searches, scan nodes
+constexpr int kStatCount = mdr::kStats;  // see md_rocm_last_stats
 
 }  // namespace
 
@@ -702,7 +702,7 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
     }
   }
   if (stats && nstats > 0) {
-    unsigned long long st4[kStatCount] = {0, 0, 0, 0, 0, 0};
+    unsigned long long st4[kStatCount] = {};
     MD_HIP(hipMemcpy(st4, s->stats.p, sizeof(st4), hipMemcpyDeviceToHost));
     for (int i = 0; i < std::min(nstats, kStatCount); ++i) stats[i] = st4[i];
   }
