@@ -204,6 +204,16 @@ int md_rocm_qe_kernel_report(int L, int accuracy, int shape, int single_precisio
  * (cpp/cuda/qe.cu:1131-1139). */
 int md_rocm_love_phase_kernel_report(int L, int single_precision, md_rocm_kernel_report *out);
 
+/* Continuity screen of completed phases, host-side (reference
+ * continuity_screen, cpp/cpu/continuity_screen.hpp:8-67): periods [T] in the
+ * caller's order, phase [B, K, T] (float32 if phase_is_f32, else float64),
+ * prediction flags [B, T] or NULL, bilateral (compare with neighbouring
+ * periods). Writes flags [B, T] (8 no bilateral neighbours, 16 envelope jump,
+ * 32 FP32 phases of adjacent modes within 16 ulp, 64 continuation review) and
+ * *suspect = 1 if any trace has 16, 32 or 64. */
+int md_rocm_continuity_screen(const double *periods, int T, const void *phase, int phase_is_f32, int B, int K,
+                              const uint32_t *prediction, int bilateral, uint8_t *flags, int *suspect);
+
 /* Device helpers (reference cpp/cuda/qe.cu:1140-1142, bindings.cpp:354-357). */
 int md_rocm_available(int device); /* 1 if the device exists and is gfx950-capable */
 int md_rocm_set_device(int device);

@@ -104,6 +104,34 @@ def love_phase_kernel_report(L, single_precision=False):
                 search_block_threads=r.search_block_threads)
 
 
+def continuity_screen(periods, phase, prediction=None, bilateral=True):
+    """Continuity flags [B, T] (uint8) of completed phases [B, K, T] and
+    whether any trace is suspect (reference _cpu.continuity_screen,
+    cpp/cpu/continuity_screen.hpp:8-67): 8 no bilateral neighbours, 16 a mode
+    outside [0.95, 1.05] of its neighbouring periods' envelope, 32 FP32 phases
+    of adjacent modes within 16 ulp, 64 continuation prediction flagged for
+    review. Host computation on the returned phases."""
+    per = np.ascontiguousarray(periods, dtype=np.float64)
+    ph = np.asarray(phase)
+    if per.ndim != 1 or ph.ndim != 3 or ph.shape[2] != per.size:
+        raise ValueError("continuity screening requires periods[T] and phases[B,K,T]")
+    f32 = ph.dtype == np.float32
+    ph = np.ascontiguousarray(ph, dtype=np.float32 if f32 else np.float64)
+    B, K, T = ph.shape
+    pred = None
+    if prediction is not None:
+        pred = np.ascontiguousarray(prediction, dtype=np.uint32)
+        if pred.shape != (B, T):
+            raise ValueError("prediction flags must have shape [B,T]")
+    out = np.zeros((B, T), dtype=np.uint8)
+    suspect = ctypes.c_int(0)
+    _native.check(_native.lib().md_rocm_continuity_screen(
+        _native.ptr(per, _native._dp), T, ph.ctypes.data_as(ctypes.c_void_p), int(f32), B, K,
+        None if pred is None else _native.ptr(pred, _native._u32p), int(bool(bilateral)),
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(suspect)))
+    return out, bool(suspect.value)
+
+
 def _rows(a, name):
     a = np.ascontiguousarray(a, dtype=np.float64)
     if a.ndim != 2:

@@ -27,7 +27,8 @@ from enum import IntEnum, IntFlag
 
 import numpy as np
 
-from . import QeSolver, BackendUnavailableError, available, bounds as _bounds, set_device, STAT_NAMES
+from . import QeSolver, BackendUnavailableError, available, bounds as _auto_bounds, set_device, STAT_NAMES
+from . import continuity_screen
 
 TRACE_UNRESOLVED_BRACKET = 1
 TRACE_RELOCATED = 2
@@ -48,7 +49,20 @@ TRACE_FP64_REPAIR = 32768
 TRACE_CONTINUATION_APPROXIMATE = 65536
 TRACE_AUTO_RELAXED = 1 << 17
 DEFAULT_ACCURACY = "auto"
+_APPROXIMATE_ACCURACIES = ("auto", "balanced", "fast")
 _C_TOL = {"auto": 1e-5, "high": 1e-6, "balanced": 1e-5, "fast": 1e-4}
+
+
+class ContinuityFlag(IntFlag):
+    """Continuity review bits (reference python/modisp/_continuity.py:6-14)."""
+    REVIEWED = 1
+    REPAIRED = 2
+    UNRESOLVED = 4
+    NO_BILATERAL = 8
+    ENVELOPE = 16
+    CLOSE_FP32 = 32
+    PREDICTION = 64
+    SEARCH = 128
 
 
 class SolveStatus(IntEnum):
@@ -91,7 +105,8 @@ class DispersionBatch:
                  accuracy=None, close_root_rtol=.005, group_velocity=None, group_velocity_flags=None,
                  prediction_flags=None, group_rtol=None, frequency_strategy="independent",
                  frequency_strategy_used="independent",
-                 frequency_strategy_reason="independent frequency search requested", anchor_stride=4):
+                 frequency_strategy_reason="independent frequency search requested", anchor_stride=4,
+                 continuity_flags=None, continuity_unresolved=False):
         self.phase_velocity = velocity
         self.precision = velocity.dtype.name
         self.period = periods.copy()
@@ -105,7 +120,7 @@ class DispersionBatch:
                                     else np.isfinite(group_velocity) & (group_velocity_flags == 0))
         self.group_velocity_valid = self.group_velocity_mask
         self.prediction_flags = prediction_flags
-        self.continuity_flags = None
+        self.continuity_flags = continuity_flags
         self.group_rtol = group_rtol
         self.frequency_strategy = frequency_strategy
         self.frequency_strategy_used = frequency_strategy_used
@@ -113,6 +128,8 @@ class DispersionBatch:
         self.anchor_stride = int(anchor_stride)
         self.approximate_search = (flags & (TRACE_APPROXIMATE_SEARCH | TRACE_CONTINUATION_APPROXIMATE)) != 0
         self.mode_index_uncertain = self.approximate_search.copy()
+        if continuity_unresolved:
+            self.mode_index_uncertain |= (continuity_flags & ContinuityFlag.UNRESOLVED) != 0
         self.close_root_mask = np.zeros(velocity.shape, dtype=bool)
         if nmodes > 1:
             gaps = np.diff(velocity, axis=1)
@@ -306,7 +323,10 @@ class Solver:
         finally:
             self._lock.release()
 
-    def _solve(self, thickness, vp, vs, rho, periods, nmodes, wave):
+    def _solve(self, thickness, vp, vs, rho, periods, nmodes, wave, _bounds=None, _row_periods=None):
+        """One solve. The private options serve the continuity review: explicit
+        search bounds [B, 2] and one period per model (`periods` then holds
+        that list's first entry, as in the reference's verifier call)."""
         started = time.perf_counter()
         if wave not in ("rayleigh", "love"):
             raise ValueError("wave must be rayleigh or love")
@@ -349,9 +369,15 @@ class Solver:
             inverse = None  # keep the caller's order for independent, all-unique grids
         frequency_used = "anchor_continuation" if continuation else "independent"
         wave_code = int(wave == "love")
-        bnd = _bounds(h, a, b, r, wave_code, self.cmin, self.cmax)
+        bnd = (_auto_bounds(h, a, b, r, wave_code, self.cmin, self.cmax) if _bounds is None
+               else np.ascontiguousarray(_bounds, dtype=np.float64))
         if not np.isfinite(bnd).all() or np.any(bnd[:, 0] <= 0):
             raise ValueError("automatic search range could not be constructed")
+        row_periods = None
+        if _row_periods is not None:
+            row_periods = np.ascontiguousarray(_row_periods, dtype=np.float64).reshape(B, 1)
+            per, inverse, continuation = per[:1], None, False
+            frequency_used, frequency_reason = "independent", "independent frequency search requested"
         capacity = min(B, self.chunk_size or B)
         key = (capacity, L, per.size, int(nmodes), self.precision, self.group_velocity, self.group_rtol,
                self.close_root_rtol, continuation, self.anchor_stride)
@@ -377,8 +403,10 @@ class Solver:
         chunks = 0
         for start in range(0, B, capacity):
             end = min(B, start + capacity)
-            out = self._engine.run(h[start:end], a[start:end], b[start:end], r[start:end], per, bnd[start:end],
-                                   self.c_tol_rel, self.f_accept, wave_code, self.cmin is None, self.profile)
+            chunk_periods = per if row_periods is None else row_periods[start:end]
+            out = self._engine.run(h[start:end], a[start:end], b[start:end], r[start:end], chunk_periods,
+                                   bnd[start:end], self.c_tol_rel, self.f_accept, wave_code, self.cmin is None,
+                                   self.profile)
             velocity[start:end], flags[start:end], part, levels[start:end], counts[start:end], extras = out
             if self.group_velocity:
                 group[start:end] = extras["group_velocity"]
@@ -397,6 +425,21 @@ class Solver:
             for k, v in part.items():
                 stats[k] = stats.get(k, 0) + int(v)
             chunks += 1
+        # Continuity screen and FP64 review of suspect traces (reference 445-460).
+        continuity = None
+        if self.accuracy in _APPROXIMATE_ACCURACIES and (self.continuity_check or continuation
+                                                         or self.accuracy == "auto"):
+            continuity, suspect = continuity_screen(per, velocity, prediction, self.continuity_check)
+            if self.accuracy == "auto":
+                failure = (flags & (TRACE_UNRESOLVED_BRACKET | TRACE_SEARCH_UNCONVERGED | TRACE_ROOT_CONFLICT)) != 0
+                if np.any(failure):
+                    continuity[failure] |= int(ContinuityFlag.SEARCH)
+                    suspect = True
+            stats.update(continuity_suspected=0, continuity_local_searches=0, continuity_full_searches=0,
+                         continuity_repaired=0, continuity_unresolved=0, continuity_evaluations=0)
+            if suspect:
+                _review(self, (h, a, b, r), per, wave, bnd, velocity, flags, levels, counts, group, gflags,
+                        continuity, stats)
         computed_periods = per.size
         if inverse is not None:
             velocity, flags, levels, counts = (x.take(inverse, axis=-1) for x in (velocity, flags, levels, counts))
@@ -404,6 +447,8 @@ class Solver:
                 group, gflags = group.take(inverse, axis=-1), gflags.take(inverse, axis=-1)
             if prediction is not None:
                 prediction = prediction.take(inverse, axis=-1)
+            if continuity is not None:
+                continuity = continuity.take(inverse, axis=-1)
         if self.cmin is not None or self.cmax is not None:
             flags |= TRACE_RANGE_TRUNCATED
         counts[(flags & TRACE_SEARCH_UNCONVERGED) != 0] = -1
@@ -417,6 +462,8 @@ class Solver:
                      frequency_strategy_reason=frequency_reason, anchor_stride=self.anchor_stride,
                      periods_requested=int(input_periods.size), periods_computed=int(computed_periods),
                      close_root_risk_traces=int(np.count_nonzero(flags & TRACE_CLOSE_ROOT_RISK)))
+        stats["evaluations"] = sum(int(stats.get(k, 0)) for k in ("scan_nodes", "fp64_rechecks", "refine_evaluations",
+                                                                  "probe_evaluations"))
         attempts = stats.get("continuation_accepted", 0) + stats.get("continuation_fallbacks", 0)
         stats["continuation_local_success_rate"] = (stats.get("continuation_accepted", 0) / attempts
                                                     if attempts else None)
@@ -430,7 +477,9 @@ class Solver:
                                  prediction_flags=prediction,
                                  group_rtol=self.group_rtol if self.group_velocity else None,
                                  frequency_strategy=self.frequency_strategy, frequency_strategy_used=frequency_used,
-                                 frequency_strategy_reason=frequency_reason, anchor_stride=self.anchor_stride)
+                                 frequency_strategy_reason=frequency_reason, anchor_stride=self.anchor_stride,
+                                 continuity_flags=continuity,
+                                 continuity_unresolved=stats.get("continuity_unresolved", 0) > 0)
         self._key = (B, L, per.size, self.nc, int(nmodes))
         self.last_result = result
         return result
@@ -472,6 +521,115 @@ class Solver:
         if self._last_stats is None:
             raise RuntimeError("call the solver first")
         return dict(self._last_stats)
+
+
+def _review(solver, models, periods, wave, bounds, velocity, trace, levels, counts, group, group_flags, flags,
+            stats):
+    """FP64 review of the traces the continuity screen flagged (reference
+    python/modisp/_continuity.py:17-205, its row-period path): every suspect
+    (model, period) is re-solved alone by an accuracy="high" solver with the
+    requested modes plus two guard roots -- first in a window cut just above
+    the observed roots, then, where that is not conclusive, in the full window
+    -- and adopted when the verifier's search is clean; a changed result
+    replaces the trace (REPAIRED), one the verifier cannot confirm is marked
+    UNRESOLVED, and every suspect is REVIEWED."""
+    reason_bits = int(ContinuityFlag.ENVELOPE | ContinuityFlag.CLOSE_FP32 | ContinuityFlag.PREDICTION
+                      | ContinuityFlag.SEARCH)
+    model_ids, freq_ids = np.nonzero(flags & reason_bits)
+    n = len(model_ids)
+    stats["continuity_suspected"] = n
+    if not n:
+        return
+    modes = velocity.shape[1]
+    guards = min(256, modes + 2)
+    settings = dict(backend="rocm", device=solver.device, accuracy="high", cmin=solver.cmin, cmax=solver.cmax,
+                    c_tol_rel=min(solver.c_tol_rel, 1e-6), f_accept=solver.f_accept, secular=solver.secular,
+                    refinement=solver.refinement, group_velocity=solver.group_velocity, group_rtol=solver.group_rtol,
+                    close_root_rtol=solver.close_root_rtol, continuity_check=False,
+                    workspace_bytes=solver.workspace_bytes, chunk_size=solver.chunk_size)
+    with Solver(**settings) as verifier:
+        def solve(rows, frequencies, limits, local):
+            selected = periods[frequencies]
+            stats["continuity_" + ("local_searches" if local else "full_searches")] += len(rows)
+            try:
+                result = verifier._solve(*(v[rows] for v in models), selected[:1], guards, wave, _bounds=limits,
+                                         _row_periods=selected)
+            except MemoryError:
+                return None
+            counters = verifier.stats()
+            stats["continuity_rocm_batches"] = stats.get("continuity_rocm_batches", 0) + counters["chunks"]
+            stats["continuity_evaluations"] += counters.get("evaluations", 0)
+            return result
+
+        def adopt(result, i, row, f, local):
+            if result is None:
+                return False
+            count = int(result.root_count[i, 0])
+            roots = result.phase_velocity[i, :, 0]
+            known = min(guards, count)
+            if (count < 0 or result.search_level[i, 0] < 0 or result.trace_flags[i, 0]
+                    & (TRACE_SEARCH_UNCONVERGED | TRACE_UNRESOLVED_BRACKET | TRACE_F_REJECTED)):
+                return False
+            if local and count < modes + 2:
+                return False
+            recovered = roots[:known].astype(velocity.dtype)
+            if not np.isfinite(recovered).all() or np.any(recovered[1:] <= recovered[:-1]):
+                return False
+            if local and (not known or roots[known - 1] >= result.search_bounds[i, 1] * (1 - 4 * verifier.c_tol_rel)):
+                return False
+            old = velocity[row, :, f]
+            new = roots[:modes].astype(old.dtype)
+            count = min(count, modes + 2)
+            alias_changed = bool(flags[row, f] & int(ContinuityFlag.CLOSE_FP32)) and not np.array_equal(
+                old, new, equal_nan=True)
+            search_failed = bool(flags[row, f] & int(ContinuityFlag.SEARCH))
+            if (search_failed or counts[row, f] != count or alias_changed
+                    or not np.allclose(old, new, rtol=solver.c_tol_rel, atol=0, equal_nan=True)):
+                old[:] = new
+                counts[row, f] = count
+                word = int(result.trace_flags[i, 0])
+                cutoff = .99 * models[2][row, -1]
+                if (word & TRACE_NEAR_CUTOFF and np.any(roots[modes:] >= cutoff)
+                        and not np.any(roots[:modes] >= cutoff)):
+                    word &= ~TRACE_NEAR_CUTOFF
+                trace[row, f] = word
+                if old.dtype == np.float32:
+                    trace[row, f] |= TRACE_FP64_REPAIR
+                levels[row, f] = result.search_level[i, 0]
+                if group is not None:
+                    group[row, :, f] = result.group_velocity[i, :modes, 0]
+                    group_flags[row, :, f] = result.group_velocity_flags[i, :modes, 0]
+                flags[row, f] |= int(ContinuityFlag.REPAIRED)
+                stats["continuity_repaired"] += 1
+            return True
+
+        # Local window: keep the original lower bound (rank) and cut the upper
+        # bound just above the observed roots when two guards were found.
+        roots = velocity[model_ids, :, freq_ids][:, :, None]
+        observed = np.where(np.isfinite(roots), roots, 0.).max(axis=(1, 2))
+        if modes > 1:
+            gaps = np.diff(roots, axis=1)
+            margin = np.where(np.isfinite(gaps), gaps, 0.).max(axis=(1, 2))
+        else:
+            margin = .1 * observed
+        limits = bounds[model_ids]
+        upper = observed + np.maximum(2.5 * margin, .02 * observed)
+        have_guards = counts[model_ids, freq_ids] >= modes + 2
+        eligible = have_guards & (guards == modes + 2) & (upper > limits[:, 0]) & (upper < limits[:, 1] * .98)
+        pending = np.ones(n, dtype=bool)
+        for local in (True, False):
+            ids = np.flatnonzero(eligible if local else pending)
+            if not ids.size:
+                continue
+            cut = limits[ids].copy()
+            if local:
+                cut[:, 1] = upper[ids]
+            result = solve(model_ids[ids], freq_ids[ids], cut, local)
+            for i, k in enumerate(ids):
+                pending[k] = not adopt(result, i, model_ids[k], freq_ids[k], local)
+        flags[model_ids[pending], freq_ids[pending]] |= int(ContinuityFlag.UNRESOLVED)
+        stats["continuity_unresolved"] += int(np.count_nonzero(pending))
+    flags[model_ids, freq_ids] |= int(ContinuityFlag.REVIEWED)
 
 
 def dispersion_batch(thickness, vp, vs, rho, periods, nmodes=5, wave="rayleigh", solver=None, names=None,

@@ -283,8 +283,13 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
         const double u = sh.ks[r];
         rank += (u < v) || (u == v && r < q);
       }
-      int ubelow = 0;
-      for (int m = 0; m < kFloor; ++m) ubelow += floor_point(m) <= v;
+      // Floor points at or below v: the closed form, corrected against the
+      // same floor_point values the merge above compares (v < hi, so m =
+      // kFloor - 1 never counts).
+      int m0 = min(kFloor - 2, max(0, int((v - lo) / dt)));
+      while (m0 < kFloor - 2 && floor_point(m0 + 1) <= v) ++m0;
+      while (m0 >= 0 && floor_point(m0) > v) --m0;
+      const int ubelow = m0 + 1;
       sh.tc[rank + ubelow] = v;
       sh.kink[rank + ubelow] = 1;
       sh.tn[rank + ubelow] = density(v);

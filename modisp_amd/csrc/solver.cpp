@@ -861,3 +861,64 @@ extern "C" int md_rocm_probe_grid(const double *thickness, const double *vp, con
   cleanup();
   return code;
 }
+
+// Continuity screen of completed phases in caller grid order (reference
+// cpp/cpu/continuity_screen.hpp:8-67, used by python/modisp/__init__.py:446-455):
+// per (model, period) a byte of ContinuityFlag bits -- 8 NO_BILATERAL (no mode
+// has finite values at both neighbouring periods), 16 ENVELOPE (a mode leaves
+// [0.95 min, 1.05 max] of its neighbours, or vanishes between them), 32
+// CLOSE_FP32 (FP32 phases of adjacent modes within 16 ulp), 64 PREDICTION
+// (continuation asked for review). Neighbours are the adjacent periods in
+// ascending order, used only when their log step is at most 0.25.
+template <class T>
+static int screen(const double *p, int Tn, const T *v, int B, int K, const uint32_t *pred, int bilateral,
+                  uint8_t *out, int *suspect) {
+  std::vector<int> order;
+  if (bilateral && Tn > 2 && !std::is_sorted(p, p + Tn)) {
+    order.resize(Tn);
+    for (int i = 0; i < Tn; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int i, int j) { return p[i] < p[j]; });
+  }
+  auto at = [&](int rank) { return order.empty() ? rank : order[rank]; };
+  int any = 0;
+  for (int j = 0; j < Tn; ++j) {
+    const int f = at(j), l = at(j > 0 ? j - 1 : j), r = at(j + 1 < Tn ? j + 1 : j);
+    const bool interior = bilateral && j > 0 && j + 1 < Tn && cont_log_step(p[r], p[l]) <= .25;
+    const bool fp32_pairs = bilateral && sizeof(T) == 4 && K > 1;
+    for (int b = 0; b < B; ++b) {
+      uint8_t flag = bilateral ? 8 : 0;
+      if (interior || fp32_pairs) {
+        for (int k = 0; k < K; ++k) {
+          const T *mode = v + (static_cast<size_t>(b) * K + k) * Tn;
+          if (interior && std::isfinite(mode[l]) && std::isfinite(mode[r])) {
+            flag &= static_cast<uint8_t>(~8u);
+            if (!std::isfinite(mode[f]) || mode[f] < T(.95) * std::min(mode[l], mode[r]) ||
+                mode[f] > T(1.05) * std::max(mode[l], mode[r]))
+              flag |= 16;
+          }
+          if (sizeof(T) == 4 && k && std::isfinite(mode[f]) && std::isfinite(mode[f - Tn])) {
+            uint32_t x, y;
+            std::memcpy(&x, mode + f, 4);
+            std::memcpy(&y, mode + f - Tn, 4);
+            if (std::max(x, y) - std::min(x, y) <= 16u) flag |= 32;
+          }
+        }
+      }
+      if (pred && (pred[static_cast<size_t>(b) * Tn + f] & mdr::kPredReview)) flag |= 64;
+      out[static_cast<size_t>(b) * Tn + f] = flag;
+      any |= (flag & (16 | 32 | 64)) != 0;
+    }
+  }
+  *suspect = any;
+  return 0;
+}
+
+extern "C" int md_rocm_continuity_screen(const double *periods, int T, const void *phase, int phase_is_f32, int B,
+                                         int K, const uint32_t *prediction, int bilateral, uint8_t *flags,
+                                         int *suspect) {
+  if (!periods || !phase || !flags || !suspect || T < 1 || B < 0 || K < 1)
+    return fail(2, "continuity screening requires periods[T] and phases[B,K,T]");
+  return phase_is_f32 ? screen(periods, T, static_cast<const float *>(phase), B, K, prediction, bilateral, flags, suspect)
+                      : screen(periods, T, static_cast<const double *>(phase), B, K, prediction, bilateral, flags,
+                               suspect);
+}
