@@ -21,6 +21,10 @@ torch.autograd binding.
 `uneven` times the ragged uneven batch (65536 models, per-model rows,
 periods and modes) with the independent search and with anchor continuation.
 
+`sweep` is the inversion-scale batch (262144 models x 31 rows, 200 periods,
+Rayleigh and Love, 4 modes, group velocity): forward roots/s, forward +
+adjoint gradient models/s and the device memory in use.
+
 `roofline` is for the dominant kernel, the FP32 Rayleigh grid scan: its
 algorithmic FLOP (secular evaluations x layer steps x 160 FLOP per Dunkin
 layer step, counted on the device) divided by its HIP-event time on the
@@ -246,6 +250,73 @@ def uneven_leg(modisp_amd, workloads, models):
     return out
 
 
+def sweep_leg(torch, modisp_amd, workloads, steps):
+    """The inversion-scale sweep (262144 models of 30 layers + half-space,
+    200 periods 0.5-100 s, Rayleigh and Love, 4 modes, group velocity):
+    forward roots/s of the FP32-scan path (both waves; timed like the main
+    steps), then forward + adjoint of the misfit sum (c - c_target)^2 over both
+    waves' phase velocities through torch.autograd (gradient models/s; targets
+    from Vs + N(0, 0.05)), and the device memory in use after each part (this
+    package's buffers persist in their solvers, so that is the peak)."""
+    import numpy as np
+    from modisp_amd.autograd import phase_velocity
+    wl = workloads.sweep_batch()
+    window = workloads.scan_window(wl.vs)
+    B = wl.B
+    free0, total = modisp_amd.memory_info()
+    cases = [DeviceCase(modisp_amd, torch, wl, window, w, "float32") for w in (0, 1)]
+    wall, _ = timed_steps(torch, cases, steps, 1)
+    roots = sum(c.valid_roots() for c in cases)
+    stats = [c.solver.last_stats() for c in cases]
+    fwd_used = total - modisp_amd.memory_info()[0]
+    for c in cases:
+        c.close()
+    del cases
+    torch.cuda.empty_cache()
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(5)
+    target_vs = np.maximum(wl.vs + rng.normal(0., 0.05, wl.vs.shape), 0.2)
+    per = torch.tensor(wl.periods, device=dev)
+    win = torch.tensor(window, device=dev)
+    L = wl.thickness.shape[1]
+    solvers = [modisp_amd.QeSolver(B, L, wl.periods.size, 4) for _ in (0, 1)]
+    with torch.no_grad():
+        tm = [torch.tensor(x, device=dev) for x in (wl.thickness, wl.vp, target_vs, wl.rho)]
+        tgt = [phase_velocity(*tm, per, win, nmodes=4, wave=w, solver=solvers[w]) for w in (0, 1)]
+    params = [torch.tensor(getattr(wl, k), device=dev, requires_grad=True) for k in ("thickness", "vp", "vs", "rho")]
+
+    def step():
+        for p_ in params:
+            p_.grad = None
+        loss = 0.
+        for w in (0, 1):
+            c = phase_velocity(*params, per, win, nmodes=4, wave=w, solver=solvers[w])
+            ok = torch.isfinite(c) & torch.isfinite(tgt[w])
+            loss = loss + ((torch.where(ok, c, 0.) - torch.where(ok, tgt[w], 0.)) ** 2).sum()
+        loss.backward()
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    gwall = time.perf_counter() - t0
+    grad_used = total - modisp_amd.memory_info()[0]
+    finite = all(bool(torch.isfinite(p_.grad).all().item()) for p_ in params)
+    for s_ in solvers:
+        s_.close()
+    return {"value": roots * steps / wall, "unit": "roots/s", "ms_per_step": wall / steps * 1e3, "roots_per_step": roots,
+            "work": {"rayleigh": stats[0], "love": stats[1]},
+            "fwd_adjoint": {"value": B * steps / gwall, "unit": "models/s", "ms_per_step": gwall / steps * 1e3,
+                            "gradients_finite": finite},
+            "hbm_in_use_bytes": {"forward": int(fwd_used), "forward_adjoint": int(grad_used),
+                                 "before": int(total - free0), "total": int(total)},
+            "config": {"workload": "sweep: 262144 models x 31 rows (Vs random walk), 200 periods 0.5-100 s, "
+                                   "Rayleigh+Love, 4 modes, group velocity; gradient w.r.t. h, vp, vs, rho",
+                       "steps": steps}}
+
+
 def traffic_per_launch():
     """HBM bytes per scan launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (scaled by a copy-kernel calibration), if present."""
@@ -268,6 +339,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the ragged uneven-batch leg")
     ap.add_argument("--uneven-models", type=int, default=65536)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the inversion-scale sweep leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every trace is independent, "
@@ -366,6 +438,12 @@ def main():
         log(f"uneven: independent {u['independent']['device_ms']:.1f} ms, continuation "
             f"{u['continuation']['device_ms']:.1f} ms ({u['continuation']['accepted']} of "
             f"{u['continuation']['targets']} targets accepted)")
+    if not args.no_sweep:
+        result["sweep"] = sweep_leg(torch, modisp_amd, workloads, 2)
+        sw = result["sweep"]
+        log(f"sweep: {sw['ms_per_step']:.1f} ms/step, {sw['value']:.3g} roots/s; fwd+adjoint "
+            f"{sw['fwd_adjoint']['ms_per_step']:.1f} ms/step, {sw['fwd_adjoint']['value']:.0f} models/s; "
+            f"HBM in use {sw['hbm_in_use_bytes']['forward_adjoint'] / 2**30:.1f} GiB")
     if not args.no_cpu:
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
         result["cpu_baseline"] = cpu_baseline(wl, window, fast, threads, min(args.cpu_sample, wl.B))

@@ -208,6 +208,7 @@ class QeSolver:
         cfg.graph = int(bool(graph))
         cfg.scan_waves = int(scan_waves)
         cfg.consumer_blocks = int(consumer_blocks)
+        cfg.refinement = int(refinement)
         self.capacity, self.L, self.nf, self.nmodes = int(B), int(L), int(nf), int(nmodes)
         self.single_precision = bool(single_precision)
         self.scan = scan

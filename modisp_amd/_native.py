@@ -24,7 +24,8 @@ class Config(ctypes.Structure):
                 ("group_rtol", ctypes.c_double), ("continuation", ctypes.c_int), ("anchor_stride", ctypes.c_int),
                 ("nc", ctypes.c_int), ("tau_floor", ctypes.c_double), ("tau_phase", ctypes.c_double),
                 ("chunk_traces", ctypes.c_int), ("grid", ctypes.c_int), ("rho", ctypes.c_double),
-                ("graph", ctypes.c_int), ("scan_waves", ctypes.c_int), ("consumer_blocks", ctypes.c_int)]
+                ("graph", ctypes.c_int), ("scan_waves", ctypes.c_int), ("consumer_blocks", ctypes.c_int),
+                ("refinement", ctypes.c_int)]
 
 
 class KernelReport(ctypes.Structure):

@@ -11,7 +11,7 @@ otherwise -- this package has no CPU path. In the reference the same
 selection falls through to its CPU backend.
 
 The engine is this package's ``QeSolver`` (FP32 density-adaptive grid scan,
-FP64 recheck, FP64 Brent refinement, analytic FP64 group velocity) for every
+FP64 recheck, FP64 bracket refinement, analytic FP64 group velocity) for every
 ``accuracy`` setting; ``accuracy`` selects the default refinement tolerance
 as in the reference. ``frequency_strategy="anchor_continuation"`` runs the
 engine's anchor continuation under the reference's conditions (not at
@@ -50,6 +50,7 @@ TRACE_CONTINUATION_APPROXIMATE = 65536
 TRACE_AUTO_RELAXED = 1 << 17
 DEFAULT_ACCURACY = "auto"
 _APPROXIMATE_ACCURACIES = ("auto", "balanced", "fast")
+_REFINEMENT = {"legacy": 0, "brent": 1, "chandrupatla": 2}   # QeSolver codes (reference qe.cu:983)
 _C_TOL = {"auto": 1e-5, "high": 1e-6, "balanced": 1e-5, "fast": 1e-4}
 
 
@@ -202,10 +203,12 @@ class Solver:
     """Reusable phase dispersion solver on the ROCm backend; use one instance
     per concurrent caller (python/modisp/__init__.py:160 for the option set).
 
-    Options the reference uses to pick among its CPU/CUDA search strategies
-    (search_strategy, sampling_rtol, secular, refinement, love_strategy,
-    adaptive_period_threshold, threads, block, nc) are validated and recorded;
-    this backend has one device method for all of them."""
+    ``refinement`` selects the bracket refinement (Brent-Dekker, Chandrupatla or
+    the legacy regula falsi, as in the reference). The options the reference
+    uses to pick among its CPU/CUDA search strategies (search_strategy,
+    sampling_rtol, secular, love_strategy, adaptive_period_threshold, threads,
+    block, nc) are validated and recorded; this backend has one device search
+    for all of them."""
 
     def __init__(self, nc=512, cmin=None, cmax=None, c_tol_rel=None, f_accept=1., block=256,
                  search_strategy="shape_adaptive", workspace_bytes=256 * 1024 ** 2, chunk_size=None, *,
@@ -380,14 +383,15 @@ class Solver:
             frequency_used, frequency_reason = "independent", "independent frequency search requested"
         capacity = min(B, self.chunk_size or B)
         key = (capacity, L, per.size, int(nmodes), self.precision, self.group_velocity, self.group_rtol,
-               self.close_root_rtol, continuation, self.anchor_stride)
+               self.close_root_rtol, continuation, self.anchor_stride, self.refinement)
         allocated = self._engine_key is None or self._engine_key[1:] != key[1:] or self._engine_key[0] < capacity
         if allocated:
             self.close()
             self._engine = QeSolver(capacity, L, per.size, int(nmodes), close_root_rtol=self.close_root_rtol,
                                     single_precision=self.precision == "float32",
                                     group_velocity=self.group_velocity, group_rtol=self.group_rtol,
-                                    continuation=continuation, anchor_stride=self.anchor_stride)
+                                    continuation=continuation, anchor_stride=self.anchor_stride,
+                                    refinement=_REFINEMENT[self.refinement])
             self._engine_key = key
         else:
             capacity = self._engine_key[0]
@@ -454,7 +458,7 @@ class Solver:
         counts[(flags & TRACE_SEARCH_UNCONVERGED) != 0] = -1
         stats.update(backend=backend, precision=self.precision, accuracy=self.accuracy,
                      strategy="rocm_adaptive_scan", requested_strategy=self.search_strategy,
-                     refinement="brent", love_strategy=self.love_strategy if wave == "love" else None,
+                     refinement=self.refinement, love_strategy=self.love_strategy if wave == "love" else None,
                      close_root_rtol=self.close_root_rtol, refinement_tolerance=self.c_tol_rel, chunks=chunks,
                      capacity_models=int(capacity), engine_allocations=int(allocated),
                      group_velocity=self.group_velocity, group_rtol=self.group_rtol,

@@ -24,8 +24,9 @@
 //   assemble_kernel    one lane per trace: merges certain FP32 signs with FP64
 //                      rechecks and trough splits into the ordered bracket
 //                      list, the root count and trace flags.
-//   refine_kernel      one lane per bracket: safeguarded Brent-Dekker in FP64
-//                      (reference cpp/common/refinement.hpp:41-63 stop rule).
+//   refine_kernel      one lane per bracket: FP64 Brent-Dekker (default),
+//                      Chandrupatla or the legacy regula falsi
+//                      (reference cpp/common/refinement.hpp:11-65 stop rules).
 //   group_kernel       one lane per root: analytic implicit group velocity
 //                      U = -dF/dk / dF/dw by forward-mode dual numbers.
 //
@@ -899,59 +900,50 @@ __global__ __launch_bounds__(kBlock) void love_phase_kernel(PipeArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Brent-Dekker refinement in FP64, one lane per (trace, root) slot.
+// FP64 bracket refinement, one lane per (trace, root) slot.
 // ---------------------------------------------------------------------------
-__device__ double brent(const PrepRow *rows, int L, int top, int wave, double omega, double a, double b, double fa,
-                        double fb, double tol, int &evals) {
-  if (fa == 0.) return a;
-  if (fb == 0.) return b;
-  const double stop = fmax(16. * 2.220446049250313e-16 * fmax(fabs(a), fabs(b)),
-                           fmin((b - a) / 64., .125 * tol * fmax(fabs(a), fabs(b))));
-  double c = b, fc = fb, d = b - a, e = d;
-  for (int j = 0; j < 160; ++j) {
-    if ((fb < 0.) == (fc < 0.)) { c = a; fc = fa; e = d = b - a; }
-    if (fabs(fc) < fabs(fb)) { a = b; fa = fb; b = c; fb = fc; c = a; fc = fa; }
-    if (fb == 0.) return b;
-    const double mid = .5 * (c - b), step = .5 * stop;
-    if (fabs(c - b) <= stop) return .5 * (b + c);
-    if (fabs(e) >= step && fabs(fa) > fabs(fb)) {
-      double s = fb / fa, p, q;
-      if (a == c) { p = 2. * mid * s; q = 1. - s; }
-      else {
-        q = fa / fc;
-        const double r = fb / fc;
-        p = s * (2. * mid * q * (q - r) - (b - a) * (r - 1.));
-        q = (q - 1.) * (r - 1.) * (s - 1.);
-      }
-      if (p > 0.) q = -q; else p = -p;
-      const double old = e;
-      e = d;
-      const double ratio = p / q;
-      if (ratio == ratio && fabs(ratio) < INFINITY && 2. * p < fmin(3. * mid * q - fabs(step * q), fabs(old * q)))
-        d = ratio;
-      else { d = mid; e = mid; }
-    } else { d = mid; e = mid; }
-    a = b; fa = fb;
-    b += fabs(d) > step ? d : (mid > 0. ? step : -step);
-    fb = secular_rows(rows, L, top, wave, omega, b);
-    ++evals;
-    if (!(fb == fb) || fabs(fb) == INFINITY) return NAN;
-  }
-  return NAN;
-}
-
-// Brent-Dekker as a per-lane state machine (the same steps and stop rule as
-// brent() above): a lane whose root converges takes its next bracket at
-// once, so every FP64 secular evaluation runs with nearly all lanes busy
-// instead of waiting for the wave's slowest bracket.
-struct BrentState {
-  double a, b, c, fa, fb, fc, d, e, stop;
-  int iter;
+// Bracket refinement as a per-lane state machine, by the solver's method
+// (reference cpp/common/refinement.hpp:11-65 and physics.hpp:224-237, the same
+// steps and stop rules): 0 regula falsi with bisection every third step and
+// near the ends (legacy), 1 Brent-Dekker (default), 2 Chandrupatla. A lane
+// whose root converges takes its next bracket at once, so every FP64 secular
+// evaluation runs with nearly all lanes busy instead of waiting for the
+// wave's slowest bracket. refine_step returns 0 when F(s.x) is needed
+// (refine_take stores it), 1 when `root` is final.
+// Brent: a, b, c (b the best point, c opposite), d, e the last steps.
+// Chandrupatla: a = x1, b = x2 (the bracket), c = x3 (the displaced end), d = t.
+// Legacy: [a, b] the bracket.
+struct RefineState {
+  double a, b, c, fa, fb, fc, d, e, stop, x, fx;
+  int iter, method;
+  bool fresh;  // fx holds F(x), not yet used
 };
 
-// Advance to the next evaluation point (returned through s.b) or finish:
-// returns 0 when F(s.b) is needed, 1 when `root` is final.
-__device__ __forceinline__ int brent_step(BrentState &s, double &root) {
+__device__ __forceinline__ double refine_stop(double a, double b, double tol) {
+  const double mag = fmax(fabs(a), fabs(b));
+  return fmax(16. * 2.220446049250313e-16 * mag, fmin((b - a) / 64., .125 * tol * mag));
+}
+
+__device__ __forceinline__ void refine_init(RefineState &s, int method, double lo, double hi, double flo, double fhi,
+                                            double tol) {
+  s.method = method;
+  s.a = lo; s.b = hi; s.fa = flo; s.fb = fhi;
+  s.c = hi; s.fc = fhi;
+  s.d = method == 2 ? .5 : hi - lo;
+  s.e = s.d;
+  s.stop = refine_stop(lo, hi, tol);
+  s.x = hi; s.fx = 0.;
+  s.iter = 0;
+  s.fresh = false;
+}
+
+__device__ __forceinline__ void refine_take(RefineState &s, double f) {
+  s.fx = f;
+  s.fresh = true;
+}
+
+__device__ __forceinline__ int brent_step(RefineState &s, double &root) {
+  if (s.fresh) { s.fb = s.fx; s.fresh = false; }
   if ((s.fb < 0.) == (s.fc < 0.)) { s.c = s.a; s.fc = s.fa; s.e = s.d = s.b - s.a; }
   if (fabs(s.fc) < fabs(s.fb)) { s.a = s.b; s.fa = s.fb; s.b = s.c; s.fb = s.fc; s.c = s.a; s.fc = s.fa; }
   if (s.fb == 0.) { root = s.b; return 1; }
@@ -978,8 +970,82 @@ __device__ __forceinline__ int brent_step(BrentState &s, double &root) {
   s.a = s.b;
   s.fa = s.fb;
   s.b += fabs(s.d) > step ? s.d : (mid > 0. ? step : -step);
+  s.x = s.b;
   ++s.iter;
   return 0;
+}
+
+__device__ __forceinline__ int chandrupatla_step(RefineState &s, double &root) {
+  if (s.fresh) {
+    s.fresh = false;
+    const double f = s.fx;
+    if (f == 0.) { root = s.x; return 1; }
+    s.c = s.b; s.fc = s.fb;
+    if ((f < 0.) == (s.fa < 0.)) { s.c = s.a; s.fc = s.fa; } else { s.b = s.a; s.fb = s.fa; }
+    s.a = s.x; s.fa = f;
+    const double dx = fabs(s.b - s.a);
+    if (dx <= s.stop) { root = .5 * (s.a + s.b); return 1; }
+    // Inverse quadratic step when the three points admit it (Chandrupatla's test).
+    const double xi = (s.a - s.b) / (s.c - s.b), phi = (s.fa - s.fb) / (s.fc - s.fb);
+    double t = .5;
+    if (xi > 0. && xi < 1. && (1. - sqrt(1. - xi)) < phi && phi < sqrt(xi)) {
+      const double alpha = (s.c - s.a) / (s.b - s.a);
+      const double z = s.fa / (s.fa - s.fb) * s.fc / (s.fc - s.fb) - alpha * s.fa / (s.fc - s.fa) * s.fb / (s.fb - s.fc);
+      if (z == z && fabs(z) < INFINITY) t = z;
+    }
+    const double limit = fmin(.5, .5 * s.stop / dx);
+    s.d = fmax(limit, fmin(1. - limit, t));
+  }
+  if (fabs(s.b - s.a) <= s.stop) { root = .5 * (s.a + s.b); return 1; }
+  if (s.iter >= 160) { root = NAN; return 1; }
+  double x = s.a + s.d * (s.b - s.a);
+  if (!(x > fmin(s.a, s.b) && x < fmax(s.a, s.b))) x = .5 * (s.a + s.b);
+  s.x = x;
+  ++s.iter;
+  return 0;
+}
+
+__device__ __forceinline__ int legacy_step(RefineState &s, double &root) {
+  if (s.fresh) {
+    s.fresh = false;
+    const double f = s.fx;
+    if (f == 0.) { root = s.x; return 1; }
+    if ((f < 0.) == (s.fa < 0.)) { s.a = s.x; s.fa = f; } else { s.b = s.x; s.fb = f; }
+  }
+  const double w = s.b - s.a;
+  if (!(s.iter < 100 && w > s.stop)) { root = w <= s.stop ? .5 * (s.a + s.b) : NAN; return 1; }
+  double x = s.a + w * (fabs(s.fa) / (fabs(s.fa) + fabs(s.fb)));
+  if (s.iter % 3 == 2 || !(fabs(x) < INFINITY) || x <= s.a + .1 * w || x >= s.b - .1 * w) x = .5 * (s.a + s.b);
+  s.x = x;
+  ++s.iter;
+  return 0;
+}
+
+__device__ __forceinline__ int refine_step(RefineState &s, double &root) {
+  if (s.method == 2) return chandrupatla_step(s, root);
+  if (s.method == 0) return legacy_step(s, root);
+  return brent_step(s, root);
+}
+
+// The same refinement run serially by one lane (continuation brackets).
+__device__ double refine_serial(int method, const PrepRow *rows, int L, int top, int wave, double omega, double a,
+                                double b, double fa, double fb, double tol, int &evals) {
+  if (method != 0) {
+    if (fa == 0.) return a;
+    if (fb == 0.) return b;
+  } else if (a == b) {
+    return a;
+  }
+  RefineState s;
+  refine_init(s, method, a, b, fa, fb, tol);
+  double root;
+  while (!refine_step(s, root)) {
+    const double f = secular_rows(rows, L, top, wave, omega, s.x);
+    ++evals;
+    if (!(f == f) || fabs(f) == INFINITY) return NAN;
+    refine_take(s, f);
+  }
+  return root;
 }
 
 __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
@@ -987,7 +1053,7 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
   const int total = ntl * a.kmax;
   const int stride = gridDim.x * kBlock;
   int q = blockIdx.x * kBlock + threadIdx.x - stride;  // advanced before each fetch
-  BrentState s{};
+  RefineState s{};
   bool active = false;
   unsigned evals = 0;
   const PrepRow *rows = nullptr;
@@ -1012,11 +1078,7 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
         if (br.fhi == 0.) { a.roots[q] = br.hi; continue; }
         // Guard roots beyond the requested modes only feed the close-root test.
         const double tol = m < mi.modes ? a.tol : fmax(a.tol, kGuardTol);
-        const double mag = fmax(fabs(br.lo), fabs(br.hi));
-        s.a = br.lo; s.b = br.hi; s.fa = br.flo; s.fb = br.fhi;
-        s.c = s.b; s.fc = s.fb; s.d = s.b - s.a; s.e = s.d;
-        s.stop = fmax(16. * 2.220446049250313e-16 * mag, fmin((s.b - s.a) / 64., .125 * tol * mag));
-        s.iter = 0;
+        refine_init(s, a.refinement, br.lo, br.hi, br.flo, br.fhi, tol);
         rows = a.rows + mi.row0;
         L = mi.L;
         top = mi.top;
@@ -1024,17 +1086,19 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
         active = true;
       }
       double root;
-      if (!brent_step(s, root)) break;  // needs F(b)
+      if (!refine_step(s, root)) break;  // needs F(s.x)
       a.roots[q] = root;
       active = false;
     }
     if (!__ballot(active)) break;
     if (!active) continue;
     ++evals;
-    s.fb = secular_rows(rows, L, top, a.wave, omega, s.b);
-    if (!(s.fb == s.fb) || fabs(s.fb) == INFINITY) {
+    const double f = secular_rows(rows, L, top, a.wave, omega, s.x);
+    if (!(f == f) || fabs(f) == INFINITY) {
       a.roots[q] = NAN;
       active = false;
+    } else {
+      refine_take(s, f);
     }
   }
   if (a.stats) wave_count(a.stats + 6, evals);
@@ -1376,7 +1440,8 @@ __global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
         }
         if (!bracket) { reason = kPredBracket; break; }
         int ev = 0;
-        const double root = brent(rows, mi.L, mi.top, a.wave, ti.omega, x0, x1, f0, f1, a.tol, ev);
+        const double root = refine_serial(a.refinement, rows, mi.L, mi.top, a.wave, ti.omega, x0, x1, f0, f1, a.tol,
+                                          ev);
         if (!(root >= x0 && root <= x1)) { reason = kPredCorrection; break; }
         if (j && root - roots[j - 1] <= a.close_rtol * fabs(root)) { reason = kPredCorrection | kPredReview; break; }
         const unsigned issue = cont_window(rows, mi.L, mi.top, a.wave, ti.omega, x0, x1, f0, f1, root, a.tol,

@@ -136,6 +136,7 @@ struct PipeArgs {
   int kmax;
   int wave;
   int fp32;
+  int refinement;  // 0 legacy regula falsi, 1 Brent-Dekker, 2 Chandrupatla
   double tol;
   double close_rtol;
   double *velocity64;
@@ -176,7 +177,7 @@ struct ContArgs {
   const PrepRow *rows;
   const int *anchors;        // anchor list: trace indices
   const TargetPlan *targets;
-  int ntargets, nanchors, kmax, wave;
+  int ntargets, nanchors, kmax, wave, refinement;
   const double *anchor_roots;  // [anchor][kmax]
   double *anchor_slopes;       // [anchor][kmax]
   double tol, close_rtol;

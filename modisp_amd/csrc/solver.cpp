@@ -152,6 +152,7 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   a.kmax = kmax;
   a.wave = wave;
   a.fp32 = s->cfg.single_precision ? 1 : 0;
+  a.refinement = s->cfg.refinement;
   a.tol = tol;
   a.close_rtol = s->cfg.close_root_rtol;
   a.velocity64 = v64;
@@ -246,6 +247,7 @@ int run_continuation(md_rocm_solver *s, int kmax, int max_rows, int wave, double
   c.nanchors = na;
   c.kmax = kmax;
   c.wave = wave;
+  c.refinement = s->cfg.refinement;
   c.anchor_roots = s->c_roots.as<double>();
   c.anchor_slopes = s->c_slopes.as<double>();
   c.tol = tol;
@@ -288,6 +290,7 @@ int check_config(const md_rocm_config &c) {
   if (c.scan_waves != 0 && c.scan_waves != 1 && c.scan_waves != 2 && c.scan_waves != 4 && c.scan_waves != 8)
     return fail(2, "scan_waves must be 0, 1, 2, 4 or 8");
   if (c.consumer_blocks < 0 || c.consumer_blocks > 64) return fail(2, "consumer_blocks must be 0..64");
+  if (c.refinement < 0 || c.refinement > 2) return fail(2, "refinement must be 0 (legacy), 1 (brent) or 2 (chandrupatla)");
   return 0;
 }
 
@@ -311,6 +314,7 @@ void md_rocm_default_config(md_rocm_config *cfg) {
   cfg->chunk_traces = 0;
   cfg->grid = 1;
   cfg->rho = 8.;
+  cfg->refinement = 1;
 }
 
 const char *md_rocm_last_error(void) { return g_error.c_str(); }

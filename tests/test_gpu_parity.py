@@ -30,12 +30,12 @@ def md():
     return modisp_amd
 
 
-def gpu_solve(md, h, vp, vs, rho, periods, bounds, nmodes, wave, scan="float32", group=True, nc=512):
+def gpu_solve(md, h, vp, vs, rho, periods, bounds, nmodes, wave, scan="float32", group=True, nc=512, refinement=1):
     h, vp, vs, rho = (np.atleast_2d(np.asarray(x, dtype=np.float64)) for x in (h, vp, vs, rho))
     B, L = h.shape
     per = np.asarray(periods, dtype=np.float64)
     nf = per.shape[-1]
-    with md.QeSolver(B, L, nf, nmodes, group_velocity=group, scan=scan, nc=nc) as s:
+    with md.QeSolver(B, L, nf, nmodes, group_velocity=group, scan=scan, nc=nc, refinement=refinement) as s:
         vel, flags, st, lev, cnt, ex = s.run(h, vp, vs, rho, per, bounds, 1e-12, 1., wave, False)
         return dict(phase_velocity=vel, trace_flags=flags, search_level=lev, root_count=cnt,
                     group_velocity=ex["group_velocity"], group_velocity_flags=ex["group_velocity_flags"],
@@ -311,3 +311,20 @@ def test_fp32_grid_signs_outside_the_band(md):
                 ref = oracle.secular(*(getattr(wl, k)[i] for k in MODEL), wl.periods[f], c[i], wave)
                 sure = np.abs(ref) > 1e-12
                 assert np.array_equal(np.signbit(ref)[sure], np.signbit(f64[i, f])[sure])
+
+
+@pytest.mark.parametrize("method", [0, 1, 2], ids=["legacy", "brent", "chandrupatla"])
+def test_refinement_methods_match_the_oracle(md, method):
+    """Each refinement the reference offers (QeSolver refinement 0 legacy
+    regula falsi, 1 Brent-Dekker, 2 Chandrupatla) against the oracle run with
+    the same method, on the precision recipe and on deep random-walk models."""
+    from modisp_amd import workloads
+    for name, wl, per in (("precision", workloads.precision_batch(B=12, seed=61), None),
+                          ("sweep", workloads.sweep_batch(B=6, seed=62), None)):
+        per = wl.periods[::4]
+        win = workloads.scan_window(wl.vs)
+        for wave in (0, 1):
+            got = gpu_solve(md, *(getattr(wl, k) for k in MODEL), per, win, 4, wave, refinement=method)
+            ref = oracle.solve(*(getattr(wl, k) for k in MODEL), per, win, 4, wave, tol=1e-12, group=2,
+                               refinement=method)
+            assert_parity(got, ref, label=f"{name} wave {wave} refinement {method}")
