@@ -186,7 +186,6 @@ struct WaveScratch {
   static constexpr int kTab = kFloor + 2 * R;
   Layer<T> lay[R];
   double tc[kTab];   // table coordinates, ascending
-  double ks[2 * R];  // layer speeds inside the window, in row order
   float tn[kTab];    // N at table points
   int pre[kTab];     // node index of table point m
   unsigned char kink[kTab];
@@ -223,81 +222,19 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
   int total = a.nc, ntab = 0;
   const double dc = (hi - lo) / double(a.nc - 1);
   if (adaptive) {
-    // Layer speeds strictly inside the window, compacted in row order.
-    int nk = 0;
-    for (int base = 0; base < L - 1; base += 64) {
-      const int j = base + lane;
-      double v0 = 0., v1 = 0.;
-      int nmine = 0;
-      if (j < L - 1 && rows[j].h > 0.) {
-        const PrepRow r = rows[j];
-        if (r.ivs > 0.) {
-          const double v = 1. / r.ivs;
-          if (v > lo && v < hi) { v0 = v; ++nmine; }
-        }
-        if (a.wave == 0) {
-          const double v = 1. / r.ivp;
-          if (v > lo && v < hi) {
-            if (nmine) v1 = v; else v0 = v;
-            ++nmine;
-          }
-        }
-      }
-      const int incl = wave_incl_scan(nmine, lane);
-      const int off = nk + incl - nmine;
-      if (nmine > 0) sh.ks[off] = v0;
-      if (nmine > 1) sh.ks[off + 1] = v1;
-      nk += __shfl(incl, 63, 64);
-    }
-    wave_sync();
-    const double dt = (hi - lo) / double(kFloor - 1);
-    auto floor_point = [&](int m) { return m == kFloor - 1 ? hi : lo + dt * double(m); };
-    auto density = [&](double c) {
-      const float k = float(omega / c), k2 = k * k;
-      float n = 0.f;
-      for (int j = top; j < L - 1; ++j) {
-        const float h = float(sh.lay[j].h);
-        const float kb = float(sh.lay[j].kb), ka = float(sh.lay[j].ka);
-        n += h * sqrtf(fmaxf(0.f, kb * kb - k2));
-        if (a.wave == 0) n += h * sqrtf(fmaxf(0.f, ka * ka - k2));
-      }
-      if (top && a.wave == 0) {  // acoustic water layer: P term only
-        const float h = float(sh.lay[0].h), ka = float(sh.lay[0].ka);
-        n += h * sqrtf(fmaxf(0.f, ka * ka - k2));
-      }
-      return n * 0.3183098861837907f;  // 1/pi
-    };
-    // Merge the kFloor (= 64, one per lane) uniform points with the layer
-    // speeds by rank.
-    {
-      const double tm = floor_point(lane);
-      int below = 0;
-      for (int r = 0; r < nk; ++r) below += sh.ks[r] < tm;
-      sh.tc[lane + below] = tm;
-      sh.kink[lane + below] = 0;
-      sh.tn[lane + below] = density(tm);
-    }
-    for (int q = lane; q < nk; q += 64) {
-      const double v = sh.ks[q];
-      int rank = 0;
-      for (int r = 0; r < nk; ++r) {
-        const double u = sh.ks[r];
-        rank += (u < v) || (u == v && r < q);
-      }
-      // Floor points at or below v: the closed form, corrected against the
-      // same floor_point values the merge above compares (v < hi, so m =
-      // kFloor - 1 never counts).
-      int m0 = min(kFloor - 2, max(0, int((v - lo) / dt)));
-      while (m0 < kFloor - 2 && floor_point(m0 + 1) <= v) ++m0;
-      while (m0 >= 0 && floor_point(m0) > v) --m0;
-      const int ubelow = m0 + 1;
-      sh.tc[rank + ubelow] = v;
-      sh.kink[rank + ubelow] = 1;
-      sh.tn[rank + ubelow] = density(v);
+    // The model's table (window floor points and layer speeds with the
+    // frequency-free mode density D, built once per model by table_kernel):
+    // N(c) = omega D(c) at this trace's frequency.
+    const TablePoint *tab = a.table + static_cast<size_t>(ti.model) * a.table_stride;
+    ntab = a.ntab[ti.model];
+    for (int m = lane; m < ntab; m += 64) {
+      const TablePoint tp = tab[m];
+      sh.tc[m] = tp.c;
+      sh.tn[m] = float(omega * double(tp.d));
+      sh.kink[m] = static_cast<unsigned char>(tp.kink);
     }
     wave_sync();
     // Density nodes per table cell; node index of every table point.
-    ntab = kFloor + nk;
     int carry = 0;
     for (int base = 0; base < ntab; base += 64) {
       const int m = base + lane;
@@ -431,6 +368,107 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
 }
 
 // R: rows the LDS scratch holds; W: wavefronts (traces) per block.
+// ---------------------------------------------------------------------------
+// Per-model scan table (the adaptive grid's skeleton): the kFloor uniform
+// points of the model's window merged by rank with every layer speed strictly
+// inside it (vs, and vp for Rayleigh), each with the frequency-free mode
+// density D(c) = (1/pi) sum_j h_j (sqrt(1/vs_j^2 - 1/c^2)+ + sqrt(1/vp_j^2 -
+// 1/c^2)+), so that N(c) = omega D(c) for every period of the model. One
+// wavefront per model; the traces of a model read the table from L2.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(kBlock) void table_kernel(const ModelInfo *models, const PrepRow *rows_all, int nmodels,
+                                                       int wave, int stride, TablePoint *table, int *ntab_out) {
+  __shared__ double ks_all[kWaves][2 * R];
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int model = blockIdx.x * kWaves + w;
+  if (model >= nmodels) return;
+  const ModelInfo mi = models[model];
+  double *ks = ks_all[w];
+  TablePoint *tab = table + static_cast<size_t>(model) * stride;
+  if (mi.empty || !(mi.hi > mi.lo)) {
+    if (lane == 0) ntab_out[model] = 0;
+    return;
+  }
+  const double lo = mi.lo, hi = mi.hi;
+  const PrepRow *rows = rows_all + mi.row0;
+  const int L = mi.L, top = mi.top;
+  // Layer speeds strictly inside the window, compacted in row order.
+  int nk = 0;
+  for (int base = 0; base < L - 1; base += 64) {
+    const int j = base + lane;
+    double v0 = 0., v1 = 0.;
+    int nmine = 0;
+    if (j < L - 1 && rows[j].h > 0.) {
+      const PrepRow r = rows[j];
+      if (r.ivs > 0.) {
+        const double v = 1. / r.ivs;
+        if (v > lo && v < hi) { v0 = v; ++nmine; }
+      }
+      if (wave == 0) {
+        const double v = 1. / r.ivp;
+        if (v > lo && v < hi) {
+          if (nmine) v1 = v; else v0 = v;
+          ++nmine;
+        }
+      }
+    }
+    const int incl = wave_incl_scan(nmine, lane);
+    const int off = nk + incl - nmine;
+    if (nmine > 0) ks[off] = v0;
+    if (nmine > 1) ks[off + 1] = v1;
+    nk += __shfl(incl, 63, 64);
+  }
+  wave_sync();
+  const double dt = (hi - lo) / double(kFloor - 1);
+  auto floor_point = [&](int m) { return m == kFloor - 1 ? hi : lo + dt * double(m); };
+  auto density = [&](double c) {
+    const double is2 = 1. / (c * c);
+    double n = 0.;
+    for (int j = top; j < L - 1; ++j) {
+      const PrepRow r = rows[j];
+      n += r.h * sqrt(fmax(0., r.ivs * r.ivs - is2));
+      if (wave == 0) n += r.h * sqrt(fmax(0., r.ivp * r.ivp - is2));
+    }
+    if (top && wave == 0) n += rows[0].h * sqrt(fmax(0., rows[0].ivp * rows[0].ivp - is2));  // water: P only
+    return n * 0.3183098861837907;  // 1/pi
+  };
+  {
+    const double tm = floor_point(lane);
+    int below = 0;
+    for (int r = 0; r < nk; ++r) below += ks[r] < tm;
+    tab[lane + below] = TablePoint{tm, float(density(tm)), 0};
+  }
+  for (int q = lane; q < nk; q += 64) {
+    const double v = ks[q];
+    int rank = 0;
+    for (int r = 0; r < nk; ++r) {
+      const double u = ks[r];
+      rank += (u < v) || (u == v && r < q);
+    }
+    // Floor points at or below v: the closed form, corrected against the same
+    // floor_point values the merge above compares (v < hi).
+    int m0 = min(kFloor - 2, max(0, int((v - lo) / dt)));
+    while (m0 < kFloor - 2 && floor_point(m0 + 1) <= v) ++m0;
+    while (m0 >= 0 && floor_point(m0) > v) --m0;
+    tab[rank + m0 + 1] = TablePoint{v, float(density(v)), 1};
+  }
+  if (lane == 0) ntab_out[model] = kFloor + nk;
+}
+
+hipError_t launch_tables(const ModelInfo *models, const PrepRow *rows, int nmodels, int max_rows, int wave, int stride,
+                         TablePoint *table, int *ntab, hipStream_t s) {
+  if (nmodels <= 0) return hipSuccess;
+  const dim3 grid((nmodels + kWaves - 1) / kWaves);
+  if (max_rows <= kAdaptRows)
+    hipLaunchKernelGGL(table_kernel<kAdaptRows>, grid, dim3(kBlock), 0, s, models, rows, nmodels, wave, stride, table,
+                       ntab);
+  else
+    hipLaunchKernelGGL(table_kernel<kMaxRows>, grid, dim3(kBlock), 0, s, models, rows, nmodels, wave, stride, table,
+                       ntab);
+  return hipGetLastError();
+}
+
 template <class T, int R, int W>
 __global__ __launch_bounds__(64 * W) void scan_kernel(PipeArgs a) {
   __shared__ WaveScratch<T, R> scratch[W];
@@ -908,14 +946,14 @@ __global__ __launch_bounds__(kBlock) void love_phase_kernel(PipeArgs a) {
 // near the ends (legacy), 1 Brent-Dekker (default), 2 Chandrupatla. A lane
 // whose root converges takes its next bracket at once, so every FP64 secular
 // evaluation runs with nearly all lanes busy instead of waiting for the
-// wave's slowest bracket. refine_step returns 0 when F(s.x) is needed
-// (refine_take stores it), 1 when `root` is final.
+// wave's slowest bracket. refine_step returns 0 when F at refine_point is
+// needed (refine_store keeps it), 1 when `root` is final.
 // Brent: a, b, c (b the best point, c opposite), d, e the last steps.
 // Chandrupatla: a = x1, b = x2 (the bracket), c = x3 (the displaced end), d = t.
 // Legacy: [a, b] the bracket.
 struct RefineState {
   double a, b, c, fa, fb, fc, d, e, stop, x, fx;
-  int iter, method;
+  int iter;
   bool fresh;  // fx holds F(x), not yet used
 };
 
@@ -926,7 +964,6 @@ __device__ __forceinline__ double refine_stop(double a, double b, double tol) {
 
 __device__ __forceinline__ void refine_init(RefineState &s, int method, double lo, double hi, double flo, double fhi,
                                             double tol) {
-  s.method = method;
   s.a = lo; s.b = hi; s.fa = flo; s.fb = fhi;
   s.c = hi; s.fc = fhi;
   s.d = method == 2 ? .5 : hi - lo;
@@ -942,8 +979,8 @@ __device__ __forceinline__ void refine_take(RefineState &s, double f) {
   s.fresh = true;
 }
 
+// Brent evaluates at s.b; refine_point / refine_store route its value to s.fb.
 __device__ __forceinline__ int brent_step(RefineState &s, double &root) {
-  if (s.fresh) { s.fb = s.fx; s.fresh = false; }
   if ((s.fb < 0.) == (s.fc < 0.)) { s.c = s.a; s.fc = s.fa; s.e = s.d = s.b - s.a; }
   if (fabs(s.fc) < fabs(s.fb)) { s.a = s.b; s.fa = s.fb; s.b = s.c; s.fb = s.fc; s.c = s.a; s.fc = s.fa; }
   if (s.fb == 0.) { root = s.b; return 1; }
@@ -970,7 +1007,6 @@ __device__ __forceinline__ int brent_step(RefineState &s, double &root) {
   s.a = s.b;
   s.fa = s.fb;
   s.b += fabs(s.d) > step ? s.d : (mid > 0. ? step : -step);
-  s.x = s.b;
   ++s.iter;
   return 0;
 }
@@ -1021,9 +1057,16 @@ __device__ __forceinline__ int legacy_step(RefineState &s, double &root) {
   return 0;
 }
 
-__device__ __forceinline__ int refine_step(RefineState &s, double &root) {
-  if (s.method == 2) return chandrupatla_step(s, root);
-  if (s.method == 0) return legacy_step(s, root);
+// The pending evaluation point, and where its value goes.
+__device__ __forceinline__ double refine_point(const RefineState &s, int method) { return method == 1 ? s.b : s.x; }
+__device__ __forceinline__ void refine_store(RefineState &s, int method, double f) {
+  if (method == 1) s.fb = f;
+  else refine_take(s, f);
+}
+
+__device__ __forceinline__ int refine_step(RefineState &s, int method, double &root) {
+  if (method == 2) return chandrupatla_step(s, root);
+  if (method == 0) return legacy_step(s, root);
   return brent_step(s, root);
 }
 
@@ -1039,15 +1082,17 @@ __device__ double refine_serial(int method, const PrepRow *rows, int L, int top,
   RefineState s;
   refine_init(s, method, a, b, fa, fb, tol);
   double root;
-  while (!refine_step(s, root)) {
-    const double f = secular_rows(rows, L, top, wave, omega, s.x);
+  while (!refine_step(s, method, root)) {
+    const double f = secular_rows(rows, L, top, wave, omega, refine_point(s, method));
     ++evals;
     if (!(f == f) || fabs(f) == INFINITY) return NAN;
-    refine_take(s, f);
+    refine_store(s, method, f);
   }
   return root;
 }
 
+// One instantiation per method (the method is the same for the whole launch).
+template <int Method>
 __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
   const int ntl = a.trace_end - a.trace_begin;
   const int total = ntl * a.kmax;
@@ -1078,7 +1123,7 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
         if (br.fhi == 0.) { a.roots[q] = br.hi; continue; }
         // Guard roots beyond the requested modes only feed the close-root test.
         const double tol = m < mi.modes ? a.tol : fmax(a.tol, kGuardTol);
-        refine_init(s, a.refinement, br.lo, br.hi, br.flo, br.fhi, tol);
+        refine_init(s, Method, br.lo, br.hi, br.flo, br.fhi, tol);
         rows = a.rows + mi.row0;
         L = mi.L;
         top = mi.top;
@@ -1086,19 +1131,19 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
         active = true;
       }
       double root;
-      if (!refine_step(s, root)) break;  // needs F(s.x)
+      if (!refine_step(s, Method, root)) break;  // needs F(s.x)
       a.roots[q] = root;
       active = false;
     }
     if (!__ballot(active)) break;
     if (!active) continue;
     ++evals;
-    const double f = secular_rows(rows, L, top, a.wave, omega, s.x);
+    const double f = secular_rows(rows, L, top, a.wave, omega, refine_point(s, Method));
     if (!(f == f) || fabs(f) == INFINITY) {
       a.roots[q] = NAN;
       active = false;
     } else {
-      refine_take(s, f);
+      refine_store(s, Method, f);
     }
   }
   if (a.stats) wave_count(a.stats + 6, evals);
@@ -1646,7 +1691,9 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
     hipLaunchKernelGGL(love_phase_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   }
   mark(4);
-  hipLaunchKernelGGL(refine_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+  if (pa.refinement == 2) hipLaunchKernelGGL(refine_kernel<2>, dim3(pb), dim3(kBlock), 0, s, pa);
+  else if (pa.refinement == 0) hipLaunchKernelGGL(refine_kernel<0>, dim3(pb), dim3(kBlock), 0, s, pa);
+  else hipLaunchKernelGGL(refine_kernel<1>, dim3(pb), dim3(kBlock), 0, s, pa);
   hipLaunchKernelGGL(finalize_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   mark(5);
   if (group) hipLaunchKernelGGL(group_kernel, dim3(pb), dim3(kBlock), 0, s, pa);

@@ -96,6 +96,14 @@ struct Event {
   int pad;            // best key of a handed-over probe (float bits)
 };
 
+// One point of a model's scan table: a window floor point or a layer speed,
+// with the frequency-free mode density there (N(c) = omega * d).
+struct TablePoint {
+  double c;
+  float d;
+  int kink;  // 1 at a layer speed (the next cell's nodes are spaced quadratically)
+};
+
 struct TraceScan {
   int nev;
   unsigned flags;
@@ -148,6 +156,9 @@ struct PipeArgs {
   uint32_t *group_flags;
   unsigned long long *stats;  // optional run statistics [kStats] (see stats_kernel)
   double *all_roots;  // optional: every (up to kmax) root per listed trace, [trace list position][kmax]
+  const TablePoint *table;  // per-model scan tables [model][table_stride] (adaptive grid)
+  const int *ntab;          // points in each model's table
+  int table_stride;
 };
 
 // ---------------------------------------------------------------------------
@@ -197,6 +208,8 @@ using ScanArgs = PipeArgs;
 // marks (optional): 7 events recorded before the scan and after the scan,
 // FP64 rescan, recheck/trough, assemble, refine/finalize and group stages.
 constexpr int kStageMarks = 7;
+hipError_t launch_tables(const ModelInfo *models, const PrepRow *rows, int nmodels, int max_rows, int wave, int stride,
+                         TablePoint *table, int *ntab, hipStream_t s);
 hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bool group, int persistent_blocks,
                            hipStream_t s, hipEvent_t *marks);
 

@@ -83,13 +83,16 @@ struct md_rocm_solver {
   hipGraphExec_t graph_exec = nullptr;
   std::vector<uintptr_t> graph_key;  // arguments the captured graph was built for
   long long cont_accepted = 0, cont_fallbacks = 0;  // continuation targets of the last run
+  int prep_models = 0;             // models prepared on the device for the current run
+  int prep_rows = 0;               // largest row count among them
+  DevBuf table, ntab;              // per-model scan tables (adaptive grid)
 
   ~md_rocm_solver() {
     for (DevBuf *b : {&rows, &models, &traces, &events, &scan, &brackets, &roots, &unc_q, &tr_q, &counters, &stats,
                       &overflow, &phase, &tape, &c_anchors, &c_targets, &c_list, &c_roots, &c_slopes,
                       &c_fallback, &c_count, &c_pred,
                       &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &out_v, &out_flags, &out_levels,
-                      &out_counts, &out_group, &out_gflags})
+                      &out_counts, &out_group, &out_gflags, &table, &ntab})
       b->release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -125,7 +128,20 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   if (int e = s->overflow.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->phase.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->stats.ensure(kStatCount * sizeof(unsigned long long))) return e;
+  // Per-model scan tables for the adaptive grid (one wavefront per model).
+  const int stride = mdr::kFloor + 2 * std::max(1, s->prep_rows - 1);
+  if (s->cfg.grid) {
+    if (int e = s->table.ensure(static_cast<size_t>(std::max(1, s->prep_models)) * stride * sizeof(mdr::TablePoint)))
+      return e;
+    if (int e = s->ntab.ensure(static_cast<size_t>(std::max(1, s->prep_models)) * sizeof(int))) return e;
+    MD_HIP(mdr::launch_tables(s->models.as<mdr::ModelInfo>(), s->rows.as<mdr::PrepRow>(), s->prep_models,
+                              s->prep_rows, wave, stride, s->table.as<mdr::TablePoint>(), s->ntab.as<int>(),
+                              s->stream));
+  }
   mdr::PipeArgs a{};
+  a.table = s->table.as<mdr::TablePoint>();
+  a.ntab = s->ntab.as<int>();
+  a.table_stride = stride;
   a.traces = list ? list : s->traces.as<mdr::TraceInfo>();
   a.all_roots = all_roots;
   a.models = s->models.as<mdr::ModelInfo>();
@@ -491,6 +507,8 @@ int device_sequence(md_rocm_solver *s, const double *thickness, const double *vp
   MD_HIP(mdr::launch_prep(thickness, vp, vs, rho, bounds, periods, row_periods, B, L, T, s->nmodes, pp,
                           s->rows.as<mdr::PrepRow>(), s->models.as<mdr::ModelInfo>(), s->traces.as<mdr::TraceInfo>(),
                           s->stream));
+  s->prep_models = B;
+  s->prep_rows = L;
   return run_pipeline(s, ntraces, s->nmodes + 2, s->L, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
                       s->cfg.group_velocity ? group : nullptr, group_flags);
 }
@@ -653,6 +671,8 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
                             s->in_bounds.as<double>(), s->in_per.as<double>(), row_periods, B, L, T, M, pp,
                             s->rows.as<mdr::PrepRow>(), s->models.as<mdr::ModelInfo>(),
                             s->traces.as<mdr::TraceInfo>(), st));
+    s->prep_models = B;
+    s->prep_rows = L;
     std::vector<int> anchors;
     std::vector<mdr::TargetPlan> targets;
     bypassed.assign(B, 0);
@@ -764,6 +784,8 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
   MD_HIP(hipMemcpyAsync(s->rows.p, rows.data(), rows.size() * sizeof(mdr::PrepRow), hipMemcpyHostToDevice, st));
   MD_HIP(hipMemcpyAsync(s->models.p, models.data(), models.size() * sizeof(mdr::ModelInfo), hipMemcpyHostToDevice,
                         st));
+  s->prep_models = B;
+  s->prep_rows = max_rows;
   MD_HIP(hipMemcpyAsync(s->traces.p, traces.data(), traces.size() * sizeof(mdr::TraceInfo), hipMemcpyHostToDevice,
                         st));
   std::vector<char> bypassed;
