@@ -157,9 +157,9 @@ int md_rocm_last_timing(md_rocm_solver *s, double *ms, int n);
  * for md_rocm_run_device): 0 traces, 1 FP64 rechecks of FP32-uncertain nodes,
  * 2 hidden-pair probes, 3 FP64 rescans of overflowed FP32 traces, 4 Love phase
  * searches, 5 secular evaluations of the grid scans, 6 FP64 evaluations of
- * the bracket refinement, 7 evaluations of the hidden-pair probes (FP32 and
- * FP64). Synchronizes the solver's stream; returns the number of values
- * written (at most 8) or -1 on error.
+ * the bracket refinement, 7 FP32 and 8 FP64 evaluations of the hidden-pair
+ * probes. Synchronizes the solver's stream; returns the number of values
+ * written (at most 9) or -1 on error.
  * No reference counterpart: the reference returns these only from run()
  * (cpp/cuda/qe.cu:815-821). */
 int md_rocm_last_stats(md_rocm_solver *s, uint64_t *stats, int n);

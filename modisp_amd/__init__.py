@@ -33,7 +33,7 @@ SUPPORTS_ROW_PERIODS = True
 SUPPORTS_GROUP_VELOCITY = True
 SUPPORTS_CONTINUATION = True
 STAT_NAMES = ("traces", "fp64_rechecks", "pair_probes", "fp64_rescans", "love_phase_searches", "scan_nodes",
-              "refine_evaluations", "probe_evaluations")
+              "refine_evaluations", "probe_evaluations", "probe_evaluations_fp64")
 
 __all__ = ["load", "QeSolver", "available", "set_device", "memory_info", "device_name", "bounds",
            "BackendUnavailableError", "ABI_VERSION", "VERSION"]

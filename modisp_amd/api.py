@@ -467,7 +467,7 @@ class Solver:
                      periods_requested=int(input_periods.size), periods_computed=int(computed_periods),
                      close_root_risk_traces=int(np.count_nonzero(flags & TRACE_CLOSE_ROOT_RISK)))
         stats["evaluations"] = sum(int(stats.get(k, 0)) for k in ("scan_nodes", "fp64_rechecks", "refine_evaluations",
-                                                                  "probe_evaluations"))
+                                                                  "probe_evaluations", "probe_evaluations_fp64"))
         attempts = stats.get("continuation_accepted", 0) + stats.get("continuation_fallbacks", 0)
         stats["continuation_local_success_rate"] = (stats.get("continuation_accepted", 0) / attempts
                                                     if attempts else None)

@@ -763,7 +763,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       golden_take(s, s.pending, f, s.dip ? ls : float(f));
     }
   }
-  if (a.stats) wave_count(a.stats + 7, evals);
+  if (a.stats) wave_count(a.stats + (fp32 ? 7 : 8), evals);
 }
 
 // ---------------------------------------------------------------------------
@@ -1548,8 +1548,8 @@ __global__ void reset_stats_kernel(unsigned long long *stats, int n) {
 // Accumulate this chunk's work counters into the run statistics:
 // [0] traces, [1] FP64 rechecks, [2] pair probes, [3] FP64 rescans,
 // [4] Love phase searches, [5] evaluated scan nodes (nodes_kernel),
-// [6] FP64 refinement evaluations, [7] probe evaluations (the consumer
-// kernels add these two themselves).
+// [6] FP64 refinement evaluations, [7] FP32 and [8] FP64 probe evaluations
+// (the consumer kernels add these themselves).
 __global__ void stats_kernel(const int *counters, unsigned long long *stats, int traces) {
   if (threadIdx.x == 0) {
     stats[0] += static_cast<unsigned long long>(traces);

@@ -19,7 +19,7 @@ namespace mdr {
 constexpr int kBlock = 256;       // lanes per scan block (4 wave64)
 constexpr int kMaxRows = 256;     // layer rows per model (reference limit, __init__.py:152)
 constexpr int kEventCap = 32;     // ascending events kept per trace
-constexpr int kStats = 8;         // run statistics (stats_kernel)
+constexpr int kStats = 9;         // run statistics (stats_kernel)
 constexpr int kFloor = 64;        // uniform table points of the adaptive grid (one per lane)
 constexpr int kAdaptRows = 64;    // rows of the common scan scratch; deeper models use a kMaxRows scratch
 constexpr int kCellMax = 1 << 16; // density nodes per table cell (cap)
