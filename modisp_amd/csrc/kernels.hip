@@ -676,7 +676,10 @@ constexpr int kProbeNeeds64 = 0x100;
 template <class T>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void trough_kernel(PipeArgs a) {
   constexpr bool fp32 = sizeof(T) == 4;
-  const int n = min(*a.trough_count, a.queue_cap);
+  // The FP64 pass of the FP32 path takes the compact list of handed-over probes.
+  const bool from_list = !fp32 && a.fp32;
+  const int n = min(from_list ? *a.handoff_count : *a.trough_count, a.queue_cap);
+  const int lane = threadIdx.x & 63;
   const int stride = gridDim.x * kBlock;
   int q = blockIdx.x * kBlock + threadIdx.x - stride;
   GoldenState s{};
@@ -696,7 +699,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       if (!active) {
         q += stride;
         if (q >= n) { exhausted = true; break; }
-        g = a.trough_queue[q];
+        g = from_list ? a.handoff_list[q] : a.trough_queue[q];
         const int tl = g / kEventCap;
         const Event &x = a.events[g];
         const int kind = x.probe & 0xff;
@@ -757,6 +760,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         x.pad = __float_as_int(s.stage == 2 ? s.kmin : INFINITY);
         x.probe |= kProbeNeeds64 | (s.it << 16);
         active = false;
+      }
+      // Append the handed-over probes to the FP64 pass's list (one atomic per
+      // wave; the lanes evaluating here are the wave's active lanes).
+      const uint64_t hm = __ballot(handed);
+      if (hm) {
+        const int leader = __ffsll(static_cast<unsigned long long>(hm)) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(a.handoff_count, __popcll(hm));
+        base = __shfl(base, leader, 64);
+        if (handed) {
+          const int slot = base + __popcll(hm & lanes_below(lane));
+          if (slot < a.queue_cap) a.handoff_list[slot] = g;
+        }
       }
     } else {
       const double f = s.sign * secular_rows(rows, L, top, a.wave, omega, s.pending, &ls);
@@ -1662,7 +1678,7 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
   auto mark = [&](int i) {
     if (marks) (void)hipEventRecord(marks[i], s);
   };
-  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, sa.counters, 4);
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, sa.counters, 5);
   // Run statistics accumulate over chunks; cleared by a kernel (not a memset
   // node) so that the sequence stays capturable.
   if (pa.stats && pa.trace_begin == 0)

@@ -127,12 +127,15 @@ struct PipeArgs {
   double *roots;
   uint32_t *unc_queue;
   uint32_t *trough_queue;
-  int *counters;  // [0] uncertain nodes, [1] pair probes, [2] overflowed FP32 traces, [3] Love phase searches
+  int *counters;  // [0] uncertain nodes, [1] pair probes, [2] overflowed FP32 traces, [3] Love phase searches,
+                  // [4] probes handed from FP32 to FP64
   int *unc_count;
   int *trough_count;
   int *overflow_count;
   int *overflow_list;  // chunk-local trace indices to rescan in FP64
   int *phase_count;
+  int *handoff_count;   // probes the FP32 probe pass handed to the FP64 pass
+  uint32_t *handoff_list;  // their event ids (reuses the recheck queue, free by then)
   int *phase_list;     // chunk-local Love traces whose brackets disagree with the phase count
   int queue_cap;
   int trace_begin, trace_end;

@@ -124,7 +124,7 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   if (int e = s->roots.ensure(static_cast<size_t>(chunk) * kmax * sizeof(double))) return e;
   if (int e = s->unc_q.ensure(qcap * sizeof(uint32_t))) return e;
   if (int e = s->tr_q.ensure(qcap * sizeof(uint32_t))) return e;
-  if (int e = s->counters.ensure(4 * sizeof(int))) return e;
+  if (int e = s->counters.ensure(5 * sizeof(int))) return e;
   if (int e = s->overflow.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->phase.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->stats.ensure(kStatCount * sizeof(unsigned long long))) return e;
@@ -158,6 +158,8 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   a.overflow_count = a.counters + 2;
   a.overflow_list = s->overflow.as<int>();
   a.phase_count = a.counters + 3;
+  a.handoff_count = a.counters + 4;
+  a.handoff_list = s->unc_q.as<uint32_t>();
   a.phase_list = s->phase.as<int>();
   a.queue_cap = static_cast<int>(qcap);
   a.nc = s->cfg.nc;
