@@ -843,7 +843,12 @@ __device__ double love_phase_dev(const PrepRow *rows, int L, int top, double ome
   if (!(k > khalf)) return NAN;
   double scale = 0.5 * rows[last].rho * rows[last].g2 * sqrt((k + khalf) * (k - khalf));
   double theta = .25 * pi;
-  auto unwrap = [](double raw, double prev) { return prev + atan2(sin(raw - prev), cos(raw - prev)); };
+  // prev + the representative of raw - prev in [-pi, pi]: the same branch as
+  // atan2(sin(raw - prev), cos(raw - prev)), by a two-term 2 pi reduction.
+  auto unwrap = [](double raw, double prev) {
+    const double d = raw - prev, n = rint(d * 0.15915494309189535);
+    return prev + fma(-n, 2.4492935982947064e-16, fma(-n, 6.283185307179586, d));
+  };
   for (int j = last - 1; j >= top; --j) {
     const PrepRow r = rows[j];
     if (r.h == 0.) continue;
@@ -851,12 +856,12 @@ __device__ double love_phase_dev(const PrepRow *rows, int L, int top, double ome
     const double q = sqrt(fabs((k + kb) * (k - kb))), next = mu * (q > 0. ? q : 1. / r.h);
     if (!(scale > 0.) || !(next > 0.) || !(fabs(scale / next) < INFINITY)) return NAN;
     double sn, cs;
-    sincos(theta, &sn, &cs);
+    sincos_phase(theta, &sn, &cs);
     theta = unwrap(atan2((scale / next) * sn, cs), theta);
     scale = next;
     if (k < kb) theta -= q * r.h;
     else {
-      sincos(theta, &sn, &cs);
+      sincos_phase(theta, &sn, &cs);
       double t, u;
       if (q == 0.) { t = sn; u = cs + sn * r.h * scale / mu; }
       else {
