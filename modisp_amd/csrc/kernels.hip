@@ -469,13 +469,25 @@ hipError_t launch_tables(const ModelInfo *models, const PrepRow *rows, int nmode
   return hipGetLastError();
 }
 
+// XCD-aware block order: the dispatcher deals blocks round-robin to the 8
+// XCDs (block b runs on XCD b % 8), each with its own L2. Renumbering so that
+// every XCD gets one contiguous range of blocks keeps the traces of a model
+// (consecutive, sharing its layer rows and scan table) on one XCD, so each
+// model's data is fetched from HBM into one L2 instead of eight. A bijection
+// of [0, nb) for any nb.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int q = nb / kXcd, r = nb % kXcd, x = b % kXcd, i = b / kXcd;
+  return x * q + min(x, r) + i;
+}
+
 template <class T, int R, int W>
 __global__ __launch_bounds__(64 * W) void scan_kernel(PipeArgs a) {
   __shared__ WaveScratch<T, R> scratch[W];
   // The wave index is uniform across the wavefront; readfirstlane lets the
   // compiler keep every per-trace quantity in scalar registers.
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int t = a.trace_begin + blockIdx.x * W + w;
+  const int t = a.trace_begin + xcd_block(blockIdx.x, gridDim.x) * W + w;
   if (t < a.trace_end) scan_wave<T, R>(a, t, scratch[w], lane);
 }
 
