@@ -200,7 +200,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
   TraceScan *ts = a.scan + tl;
   constexpr bool fp32 = sizeof(T) == 4;
   if (mi.empty) {
-    if (lane == 0) *ts = TraceScan{0, 0u, mi.empty, 0, 0, 0};
+    if (lane == 0) *ts = TraceScan{0, 0u, mi.empty, 0, 0, 0, 0u, 0u};
     return;
   }
   const double lo = mi.lo, hi = mi.hi;
@@ -208,7 +208,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
   // (reference limit: max Vs / 16); such windows go straight to the FP64 grid.
   if (fp32 && lo < mi.stable_low) {
     if (lane == 0) {
-      *ts = TraceScan{0, kTraceOverflow, 0, 0, 0, 0};
+      *ts = TraceScan{0, kTraceOverflow, 0, 0, 0, 0, 0u, 0u};
       a.overflow_list[atomicAdd(a.overflow_count, 1)] = tl;
     }
     return;
@@ -268,6 +268,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
   const double hi_guard = mi.cutoff_guard;
   const float theta = kTroughMax;
   int nb = 0, ne = 0;
+  unsigned umask = 0u, pmask = 0u;
   bool overflow = false;
   int evaluated = 0;
   for (int base = 0;; base += 62) {
@@ -335,6 +336,21 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
       break;
     }
     const bool mine = (em >> lane) & 1;
+    {
+      // Event bits of this batch for the queue compaction: the lanes of em in
+      // order are slots ne, ne + 1, ...; compress the uncertain / probed lane
+      // ballots onto those slots (wave-uniform, scalar work).
+      const uint64_t ub = __ballot(kind == kUncertain) & em, pb = __ballot(probe != 0) & em;
+      unsigned cu = 0u, cp = 0u;
+      int i = 0;
+      for (uint64_t m = em; m; m &= m - 1ull, ++i) {
+        const uint64_t low = m & (~m + 1ull);
+        if (ub & low) cu |= 1u << i;
+        if (pb & low) cp |= 1u << i;
+      }
+      umask |= cu << ne;
+      pmask |= cp << ne;
+    }
     if (mine) {
       const int slot = ne + __popcll(em & lanes_below(lane));
       Event x;
@@ -360,7 +376,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
     if (done) break;
   }
   if (lane == 0) {
-    *ts = TraceScan{overflow ? 0 : ne, overflow ? kTraceOverflow : 0u, 0, nb, 0, evaluated};
+    *ts = TraceScan{overflow ? 0 : ne, overflow ? kTraceOverflow : 0u, 0, nb, 0, evaluated, umask, pmask};
     // An FP32 trace whose events overflow is rescanned on the FP64 grid,
     // which has no uncertain nodes.
     if (overflow && fp32) a.overflow_list[atomicAdd(a.overflow_count, 1)] = tl;
@@ -514,16 +530,12 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(PipeArgs a) {
   const int tl = blockIdx.x * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool live = tl < a.trace_end - a.trace_begin;
-  int nu = 0, np = 0, nev = 0;
-  const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
+  unsigned um = 0u, pm = 0u;
   if (live) {
     const TraceScan ts = a.scan[tl];
-    if (ts.status == 0 && !(ts.flags & kTraceOverflow)) nev = ts.nev;
-    for (int e = 0; e < nev; ++e) {
-      nu += ev[e].kind == kUncertain;
-      np += ev[e].probe != 0;
-    }
+    if (ts.status == 0 && !(ts.flags & kTraceOverflow)) { um = ts.umask; pm = ts.pmask; }
   }
+  const int nu = __popc(um), np = __popc(pm);
   const int iu = wave_incl_scan(nu, lane), ip = wave_incl_scan(np, lane);
   if (lane == 63) { part[0][w] = iu; part[1][w] = ip; }
   __syncthreads();
@@ -539,11 +551,9 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(PipeArgs a) {
   }
   __syncthreads();
   int qu = base[0] + part[0][w] + iu - nu, qp = base[1] + part[1][w] + ip - np;
-  for (int e = 0; e < nev; ++e) {
-    const uint32_t g = static_cast<uint32_t>(tl * kEventCap + e);
-    if (ev[e].kind == kUncertain) a.unc_queue[qu++] = g;
-    if (ev[e].probe != 0) a.trough_queue[qp++] = g;
-  }
+  const uint32_t g0 = static_cast<uint32_t>(tl) * kEventCap;
+  for (; um; um &= um - 1u) a.unc_queue[qu++] = g0 + __builtin_ctz(um);
+  for (; pm; pm &= pm - 1u) a.trough_queue[qp++] = g0 + __builtin_ctz(pm);
 }
 
 // ---------------------------------------------------------------------------

@@ -111,6 +111,7 @@ struct TraceScan {
   int certain;  // certain sign changes seen by the scan
   int count;    // roots found (<= K), -1 unresolved
   int nodes;    // secular evaluations of the trace's scan (lane slots, batch overlaps included)
+  unsigned umask, pmask;  // events (bit e) that are FP32-uncertain / probed for a hidden pair
 };
 
 struct Bracket {
