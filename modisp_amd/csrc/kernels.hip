@@ -3,36 +3,43 @@
 // A trace is one (model, period) pair. For every trace the pipeline finds the
 // first K = modes + 2 phase-velocity roots of the secular function F(c) in the
 // trace's window [lo, hi] (the reference's internal guard roots,
-// cpp/cpu/solver.cpp:418), refines the first `modes`, and optionally their
-// group velocities:
+// cpp/cpu/solver.cpp:418), refines them in FP64, and optionally computes the
+// group velocities of the first `modes`:
 //
-//   scan_kernel<T,S>   one 256-lane block per trace; nc = 256*S uniform
-//                      c-samples, one lane per sample (S per lane), layer
-//                      constants staged in LDS; block-wide prefix scans
-//                      classify and compact the ascending events up to the
-//                      K-th certain sign change into the trace's event list:
-//                      certain brackets, FP32-uncertain nodes (|F| < tau, or
-//                      inside the branch-point and low-velocity guards) and
-//                      same-sign troughs; uncertain nodes and troughs are
-//                      appended to device work queues.
-//   rescan_kernel      FP64 rescan (persistent) of FP32 traces whose event
-//                      list overflowed: no node of an FP64 grid is uncertain.
-//   recheck_kernel     FP64 re-evaluation of every queued uncertain node,
-//                      one lane per node (dense across traces).
-//   trough_kernel      FP64 golden-section probe of each queued trough for a
-//                      hidden root pair (one lane per trough).
-//   assemble_kernel    one lane per trace: merges certain FP32 signs with FP64
-//                      rechecks and trough splits into the ordered bracket
-//                      list, the root count and trace flags.
-//   refine_kernel      one lane per bracket: FP64 Brent-Dekker (default),
-//                      Chandrupatla or the legacy regula falsi
-//                      (reference cpp/common/refinement.hpp:11-65 stop rules).
-//   group_kernel       one lane per root: analytic implicit group velocity
+//   table_kernel       one wavefront per model: the adaptive grid's table (the
+//                      window's floor points merged with the layer speeds, with
+//                      the frequency-free mode density).
+//   scan_kernel        one wavefront per trace, W traces per block in an
+//                      XCD-aware block order; layer constants in LDS; nodes
+//                      (density-adaptive or uniform) evaluated 64 at a time in
+//                      FP32 or FP64; ballots classify certain sign changes,
+//                      FP32-uncertain nodes and hidden-pair candidates
+//                      (troughs, scale dips) into the trace's event list, up to
+//                      the K-th certain sign change.
+//   rescan_kernel      FP64 rescan (persistent) of FP32 traces whose event list
+//                      overflowed or whose window needs FP64.
+//   compact_kernel     recheck and probe queues from the per-trace event masks
+//                      (one atomic per block).
+//   recheck_kernel     FP64 value of every uncertain node (dense lanes).
+//   unc_probe_kernel   trough test at the rechecked nodes.
+//   trough_kernel      golden-section probes of the candidates, one search per
+//                      lane as a state machine: an FP32 pass, and an FP64 pass
+//                      over the searches it handed over (or every probe of the
+//                      FP64 path); a point of opposite sign splits the cell.
+//   assemble_kernel    one lane per trace: ordered brackets and the root count.
+//   love_count_kernel  Love: the exact phase count verifies each trace;
+//   love_phase_kernel  a disagreeing trace is re-bracketed by phase bisection.
+//   refine_kernel      one bracket per lane as a state machine: FP64
+//                      Brent-Dekker (default), Chandrupatla or the legacy
+//                      regula falsi (reference cpp/common/refinement.hpp:11-65).
+//   finalize_kernel    outputs in the reference layout, trace flags.
+//   group_kernel       mode-major lanes: analytic implicit group velocity
 //                      U = -dF/dk / dF/dw by forward-mode dual numbers.
+//   slope/anchor_flag/continuation/gather kernels: anchor continuation.
 //
-// The persistent kernels (recheck, trough, refine, group) read their work
-// counts from device memory, so the whole sequence is launched back to back on
-// one stream with no host synchronisation and can be captured in a hipGraph.
+// Work counts live in device memory and the consumer kernels are persistent
+// grid-stride kernels, so the sequence runs back to back on one stream with no
+// host synchronisation and can be captured in a hipGraph.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "pipeline.hpp"
