@@ -314,6 +314,7 @@ struct AdjointArgs {
   double *gh, *gvp, *gvs, *grho;    // [B, L] sum over roots of weight * dc/dp
   double *tape;                     // [L * 6 * tape_lanes], lane-interleaved
   int B, L, T, modes, row_periods, wave, tape_lanes;
+  int group;                        // models a block takes at once (LDS: group * 4 * L doubles)
 };
 hipError_t launch_adjoint(const AdjointArgs &a, int blocks, hipStream_t s);
 // Compiled resources of the kernel that scans a model of L rows (which = 0)

@@ -16,7 +16,8 @@
 //    of its largest magnitude, which changes neither the sign nor the zeros
 //    of F and costs one hardware reciprocal instead of five divisions.
 // The same template serves float (grid scans) and double (brackets,
-// refinement); Dual2 (secular_dual.hpp) reuses it for analytic derivatives.
+// refinement); Dual2 (below) reuses it for analytic derivatives, and the
+// root adjoint's reverse-mode layer derivative closes the file.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -177,8 +178,8 @@ template <class T>
 struct Wave {
   T c, w, x, att;
 };
-// r and 1/r from r^2: one hardware reciprocal square root for float, the
-// correctly rounded sqrt and division for double.
+// r and 1/r from r^2: one hardware reciprocal square root for float; for
+// double, that estimate refined by Newton steps (rsq_newton).
 MDR_HD void root_pair(float r2, float *r, float *ir) {
 #if defined(__HIP_DEVICE_COMPILE__)
   const float q = __builtin_amdgcn_rsqf(r2);
@@ -252,15 +253,12 @@ struct Compound {
 };
 
 template <class T>
-MDR_HD Compound<T> rayleigh_compound(const Layer<T> &ly, T k, T k2) {
-  const Wave<T> P = wave_factor(k, ly.ka, ly.h);
-  const Wave<T> S = wave_factor(k, ly.kb, ly.h);
+MDR_HD Compound<T> compound_from_waves(const Wave<T> &P, const Wave<T> &S, T gk, T rho, T irho, T irho2, T k2) {
   const T a0 = P.att * S.att;
   const T cpcq = P.c * S.c, cpy = P.c * S.w, cpz = P.c * S.x, cqw = S.c * P.w, cqx = S.c * P.x;
   const T xy = P.x * S.w, xz = P.x * S.x, wy = P.w * S.w, wz = P.w * S.x;
-  const T gk = ly.gk, gam = gk * k2, gamm1 = gam - T(1), twgm1 = gam + gamm1;
+  const T gam = gk * k2, gamm1 = gam - T(1), twgm1 = gam + gamm1;
   const T gmgmk = gam * gk, gmgm1 = gam * gamm1, gm1sq = gamm1 * gamm1;
-  const T rho = ly.rho, irho = ly.irho, irho2 = ly.irho2;
   const T a0pq = a0 - cpcq, k4 = k2 * k2;
   Compound<T> q;
   q.A = cpcq - T(2) * gmgm1 * a0pq - gmgmk * xz - k2 * gm1sq * wy;
@@ -280,6 +278,11 @@ MDR_HD Compound<T> rayleigh_compound(const Layer<T> &ly, T k, T k2) {
   q.K = a0 + T(2) * (cpcq - q.A);
   q.t = T(-2) * k2;
   return q;
+}
+template <class T>
+MDR_HD Compound<T> rayleigh_compound(const Layer<T> &ly, T k, T k2) {
+  return compound_from_waves(wave_factor(k, ly.ka, ly.h), wave_factor(k, ly.kb, ly.h), ly.gk, ly.rho, ly.irho,
+                             ly.irho2, k2);
 }
 
 // n = C e (the propagation step, FMA contraction of the 15 entries).
@@ -505,4 +508,130 @@ MDR_HD Dual2 rescale(Dual2 *e) {
   }
   return Dual2(1.);
 }
+}  // namespace mdr
+
+// ---------------------------------------------------------------------------
+// Reverse-mode derivative of one Rayleigh layer for the root adjoint: the
+// bilinear form S = l^T C e (C the layer's compound matrix at a fixed k) and
+// its derivatives with respect to the layer constants h, ka, kb, gk and rho
+// (rho also through 1/rho and 1/rho^2). S is linear in ten products of the
+// P and S wave factors (a0 = att_P att_S, cpcq = c_P c_S, ...); their
+// coefficients beta depend on gk, rho and the entry weights only, so the
+// gk and rho derivatives are sum_b beta_b' b (beta in duals of gk and rho),
+// and the h, ka, kb derivatives go through the adjoints of the wave factors.
+// ---------------------------------------------------------------------------
+namespace mdr {
+
+// Wave factor with its derivatives along h and along kx (same branches as
+// wave_factor; at k == kx only the h derivative of w = h is kept).
+struct WaveGrad {
+  Wave<double> v, dh, dkx;
+};
+MDR_HD WaveGrad wave_factor_grad(double k, double kx, double h) {
+  WaveGrad o;
+  double r, ir;
+  root_pair((k + kx) * std::fabs(k - kx), &r, &ir);
+  const double p = r * h;
+  if (k < kx) {  // p = r h, r^2 = kx^2 - k^2: dp/dh = r, dr/dkx = kx / r
+    double s, c;
+    m_sincos(p, &s, &c);
+    o.v = {c, s * ir, -r * s, 1.};
+    o.dh = {-s * r, c, -r * r * c, 0.};
+    const double drk = kx * ir, dpk = h * drk;
+    o.dkx = {-s * dpk, (c * dpk - s * ir * drk) * ir, -(drk * s + r * c * dpk), 0.};
+  } else if (k == kx) {
+    o.v = {1., h, 0., 1.};
+    o.dh = {0., 1., 0., 0.};
+    o.dkx = {0., 0., 0., 0.};
+  } else {  // r^2 = k^2 - kx^2: dr/dkx = -kx / r; with f = e^-2p: c' = -f, s' = f, att' = -att
+    const double e = m_exp(-p), f = e * e, s = (1. - f) * 0.5;
+    o.v = {(1. + f) * 0.5, s * ir, r * s, e};
+    o.dh = {-f * r, f, r * r * f, -e * r};
+    const double drk = -kx * ir, dpk = h * drk;
+    o.dkx = {-f * dpk, (f * dpk - s * ir * drk) * ir, drk * s + r * f * dpk, -e * dpk};
+  }
+  return o;
+}
+
+// Weights of the bilinear form l^T C e over the distinct entries of the
+// compound matrix laid out as [A E tJ F O; D B tH N F; I G K H J;
+// C M tG B E; Lq C tI D A] (t = -2 k^2 folded in): l^T C e = sum_X w_X X.
+struct CompoundWeights {
+  double A, B, C, D, E, F, G, H, I, J, K, Lq, M, N, O;
+};
+MDR_HD CompoundWeights compound_weights(const double l[5], const double e[5], double t) {
+  const double te2 = t * e[2];
+  CompoundWeights w;
+  w.A = std::fma(l[0], e[0], l[4] * e[4]);
+  w.E = std::fma(l[0], e[1], l[3] * e[4]);
+  w.J = std::fma(l[0], te2, l[2] * e[4]);
+  w.F = std::fma(l[0], e[3], l[1] * e[4]);
+  w.O = l[0] * e[4];
+  w.D = std::fma(l[1], e[0], l[4] * e[3]);
+  w.B = std::fma(l[1], e[1], l[3] * e[3]);
+  w.H = std::fma(l[1], te2, l[2] * e[3]);
+  w.N = l[1] * e[3];
+  w.I = std::fma(l[2], e[0], l[4] * te2);
+  w.G = std::fma(l[2], e[1], l[3] * te2);
+  w.K = l[2] * e[2];
+  w.C = std::fma(l[3], e[0], l[4] * e[1]);
+  w.M = l[3] * e[1];
+  w.Lq = l[4] * e[0];
+  return w;
+}
+
+// dS/d(h, ka, kb, gk, rho) of S = l^T C e for the layer constants given;
+// l is replaced by l^T C (the adjoint step, unscaled).
+struct LayerGrad {
+  double h, ka, kb, gk, rho;
+};
+MDR_HD LayerGrad rayleigh_layer_adjoint(double h, double ka, double kb, double gk, double rho, double irho,
+                                        double irho2, double k, double l[5], const double e[5]) {
+  const WaveGrad P = wave_factor_grad(k, ka, h), S = wave_factor_grad(k, kb, h);
+  const double k2 = k * k, k4 = k2 * k2;
+  const CompoundWeights w = compound_weights(l, e, -2. * k2);
+  // the ten products
+  const double a0 = P.v.att * S.v.att, cpcq = P.v.c * S.v.c, cpy = P.v.c * S.v.w, cpz = P.v.c * S.v.x;
+  const double cqw = S.v.c * P.v.w, cqx = S.v.c * P.v.x, xy = P.v.x * S.v.w, xz = P.v.x * S.v.x;
+  const double wy = P.v.w * S.v.w, wz = P.v.w * S.v.x;
+  // their coefficients, in duals of (gk, rho)
+  const Dual2 g(gk, 1., 0.), rh(rho, 0., 1.), irh(irho, 0., -irho * irho), irh2(irho2, 0., -2. * irho2 * irho);
+  const Dual2 gam = g * k2, gm1 = gam - Dual2(1.), tw = gam + gm1, gg = gam * g, gm = gam * gm1, g2s = gm1 * gm1;
+  const Dual2 rr = rh * rh, ggr = gg * rh, g2r = g2s * rh;
+  const double ap = w.A - 2. * w.K;  // K = a0 + 2 (cpcq - A)
+  const Dual2 Q = -(gm * (2. * ap) + tw * irh * w.I + irh2 * (2. * k2 * w.Lq) + gg * g2s * rr * (2. * w.O) +
+                    g * gm1 * tw * rh * w.J);
+  const Dual2 b_a0 = Q + Dual2(w.K);
+  const Dual2 b_cpcq = Dual2(ap + 2. * w.K + w.B) - Q;
+  const Dual2 b_cpy = irh * (k2 * w.D) + g2r * w.F + gm1 * w.H;
+  const Dual2 b_cqx = -(irh * w.D + ggr * w.F + g * w.H);
+  const Dual2 b_cpz = irh * w.C + ggr * w.E + g * w.G;
+  const Dual2 b_cqw = -(irh * (k2 * w.C) + g2r * w.E + gm1 * w.G);
+  const Dual2 b_xz = -(gg * ap + g * irh * w.I + irh2 * w.Lq + gg * gg * rr * w.O + gg * g * rh * w.J);
+  const Dual2 b_wy = -(g2s * (k2 * ap) + gm1 * irh * (k2 * w.I) + irh2 * (k4 * w.Lq) + g2s * g2s * rr * w.O +
+                       gm1 * g2r * w.J);
+  const double b_wz = -w.M, b_xy = -w.N;
+  LayerGrad d;
+  d.gk = b_a0.a * a0 + b_cpcq.a * cpcq + b_cpy.a * cpy + b_cqx.a * cqx + b_cpz.a * cpz + b_cqw.a * cqw +
+         b_xz.a * xz + b_wy.a * wy;
+  d.rho = b_a0.b * a0 + b_cpcq.b * cpcq + b_cpy.b * cpy + b_cqx.b * cqx + b_cpz.b * cpz + b_cqw.b * cqw +
+          b_xz.b * xz + b_wy.b * wy;
+  // adjoints of the wave factors
+  const double pc = b_cpcq.v * S.v.c + b_cpy.v * S.v.w + b_cpz.v * S.v.x;
+  const double pw = b_cqw.v * S.v.c + b_wy.v * S.v.w + b_wz * S.v.x;
+  const double px = b_cqx.v * S.v.c + b_xy * S.v.w + b_xz.v * S.v.x;
+  const double pa = b_a0.v * S.v.att;
+  const double sc = b_cpcq.v * P.v.c + b_cqw.v * P.v.w + b_cqx.v * P.v.x;
+  const double sw = b_cpy.v * P.v.c + b_xy * P.v.x + b_wy.v * P.v.w;
+  const double sx = b_cpz.v * P.v.c + b_xz.v * P.v.x + b_wz * P.v.w;
+  const double sa = b_a0.v * P.v.att;
+  d.ka = pc * P.dkx.c + pw * P.dkx.w + px * P.dkx.x + pa * P.dkx.att;
+  d.kb = sc * S.dkx.c + sw * S.dkx.w + sx * S.dkx.x + sa * S.dkx.att;
+  d.h = pc * P.dh.c + pw * P.dh.w + px * P.dh.x + pa * P.dh.att + sc * S.dh.c + sw * S.dh.w + sx * S.dh.x +
+        sa * S.dh.att;
+  // l <- l^T C from the entries' values
+  compound_apply_transpose(compound_from_waves(P.v, S.v, gk, rho, irho, irho2, k2), l);
+  return d;
+}
+
 }  // namespace mdr

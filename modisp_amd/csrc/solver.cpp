@@ -482,7 +482,11 @@ int md_rocm_adjoint(md_rocm_solver *s, const double *thickness, const double *vp
   if (!thickness || !vp || !vs || !rho || !periods || !roots || !weight || !grad_thickness || !grad_vp ||
       !grad_vs || !grad_rho)
     return fail(2, "null device pointer");
-  const int blocks = std::max(1, std::min(B, s->cus * 8));
+  // A block takes `group` models at once so that their roots fill its 64
+  // lanes; the group shrinks as the batch falls below the 8 blocks per CU
+  // that keep the device full, and its LDS stays within 16 KiB.
+  const int group = std::max(1, std::min({8, B / (s->cus * 8), 512 / s->L}));
+  const int blocks = std::max(1, std::min((B + group - 1) / group, s->cus * 8));
   const int lanes = blocks * 64;
   if (int e = s->tape.ensure(static_cast<size_t>(lanes) * s->L * 6 * sizeof(double))) return e;
   mdr::AdjointArgs a{};
@@ -492,6 +496,7 @@ int md_rocm_adjoint(md_rocm_solver *s, const double *thickness, const double *vp
   a.tape = s->tape.as<double>();
   a.B = B; a.L = s->L; a.T = s->nf; a.modes = s->nmodes; a.row_periods = row_periods ? 1 : 0; a.wave = wave;
   a.tape_lanes = lanes;
+  a.group = group;
   const hipError_t err = mdr::launch_adjoint(a, blocks, s->stream);
   if (err != hipSuccess) return fail(1, std::string("adjoint launch: ") + hipGetErrorString(err));
   return 0;
