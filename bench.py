@@ -13,6 +13,11 @@ per second of the FP32-scan path. The same steps are timed for the plain
 FP64 path (identical grid and pipeline, every scan evaluation in FP64) and,
 for reference, for the literal 512-sample uniform grid in both precisions.
 
+`graph` is the same device-resident step replayed from one captured hipGraph
+per wave type. `end_to_end` follows the reference's own timing protocol:
+Solver calls on host arrays (copies and host result assembly included), median
+of the steady calls, valid roots = status-0 cells.
+
 `fwd_adjoint` is the second half of the metric: models/s of forward root
 solve + misfit loss + adjoint backward on the differentiation workload
 (4096 models x 21 rows, 50 periods, Rayleigh, 3 modes) through the
@@ -59,7 +64,7 @@ def log(msg):
 class DeviceCase:
     """One wave type of the workload resident on the device, with its solver."""
 
-    def __init__(self, modisp_amd, torch, wl, window, wave, scan, grid="adaptive", group=True):
+    def __init__(self, modisp_amd, torch, wl, window, wave, scan, grid="adaptive", group=True, graph=False):
         dev = torch.device("cuda", 0)
         self.B, self.L = wl.thickness.shape
         self.T = wl.periods.size
@@ -76,7 +81,7 @@ class DeviceCase:
         self.group = torch.empty(n, dtype=torch.float64, device=dev) if group else None
         self.gflags = torch.empty(n, dtype=torch.int32, device=dev) if group else None
         self.solver = modisp_amd.QeSolver(self.B, self.L, self.T, self.M, group_velocity=group, scan=scan, grid=grid,
-                                          nc=512)
+                                          nc=512, graph=graph)
         self.ptrs = dict(thickness=self.h.data_ptr(), vp=self.vp.data_ptr(), vs=self.vs.data_ptr(),
                          rho=self.rho.data_ptr(), periods=self.periods.data_ptr(), bounds=self.bounds.data_ptr(),
                          velocity=self.velocity.data_ptr(), flags=self.flags.data_ptr(),
@@ -158,6 +163,44 @@ def cpu_baseline(wl, window, gpu_cases, threads, sample):
                       f"{gpu_cases[0].T} periods, analytic group velocity: {roots} roots in {elapsed:.2f} s "
                       f"on {threads} threads",
             "gpu_parity_on_sample": agree}
+
+
+def end_to_end_leg(wl, steps):
+    """The reference's own timing protocol (tools/benchmark_backends.py:140-200,
+    556-559) on the same workload: Solver(precision="float32", ...) called
+    with host arrays per wave type, one first call, then `steps` steady calls;
+    valid roots are status-0 cells and roots/s = the two waves' valid roots /
+    the sum of their median call times. Includes the host-to-device copies of
+    the models, the device-to-host copies of every result array and the host
+    result assembly (status, masks, flags), so it is the PCIe-inclusive rate,
+    not `value`. The tolerance is the device leg's (1e-12 relative) and the
+    window the front end's own (reference bounds)."""
+    import numpy as np
+    from modisp_amd import api
+    opts = dict(backend="rocm", precision="float32", secular="fast", accuracy="high", c_tol_rel=1e-12,
+                search_strategy="shape_adaptive", refinement="brent", love_strategy="phase", close_root_rtol=.005,
+                f_accept=1., group_velocity=True, workspace_bytes=1 << 30)
+    out, roots, seconds = {}, 0, 0.0
+    for wave in ("rayleigh", "love"):
+        with api.Solver(**opts) as solver:
+            call = lambda: solver(wl.thickness, wl.vp, wl.vs, wl.rho, wl.periods, nmodes=int(wl.nmodes), wave=wave)
+            t0 = time.perf_counter()
+            res = call()
+            first = time.perf_counter() - t0
+            times = []
+            for _ in range(steps):
+                t0 = time.perf_counter()
+                res = call()
+                times.append(time.perf_counter() - t0)
+            valid = int(np.count_nonzero(res.status == 0))
+            med = float(np.median(times))
+            out[wave] = {"valid_roots": valid, "median_s": med, "first_s": first,
+                         "iqr_s": float(np.subtract(*np.percentile(times, [75, 25])))}
+            roots += valid
+            seconds += med
+    out.update({"value": roots / seconds, "unit": "roots/s", "steady_calls": steps,
+                "options": {k: v for k, v in opts.items() if k != "workspace_bytes"}})
+    return out
 
 
 def gradient_leg(torch, modisp_amd, workloads, steps, warmup):
@@ -340,6 +383,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the ragged uneven-batch leg")
     ap.add_argument("--uneven-models", type=int, default=65536)
     ap.add_argument("--no-sweep", action="store_true", help="skip the inversion-scale sweep leg")
+    ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay leg")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end Solver leg (host arrays)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every trace is independent, "
@@ -361,8 +406,8 @@ def main():
     L = wl.thickness.shape[1]
     log(f"workload {wl.name}: B={wl.B} L={L} T={wl.periods.size} modes={wl.nmodes}")
 
-    def run_path(scan, grid):
-        cases = [DeviceCase(modisp_amd, torch, wl, window, w, scan, grid) for w in (0, 1)]
+    def run_path(scan, grid, graph=False):
+        cases = [DeviceCase(modisp_amd, torch, wl, window, w, scan, grid, graph=graph) for w in (0, 1)]
         wall, _ = timed_steps(torch, cases, args.steps, args.warmup)
         roots = sum(c.valid_roots() for c in cases)
         # Stage times and work counters from one more pass of each wave type on
@@ -428,6 +473,17 @@ def main():
     }
     if uniform:
         result["uniform512"]["speedup_fp32_over_fp64"] = uniform["float32"]["value"] / uniform["float64"]["value"]
+    if not args.no_graph:
+        # The same device-resident step replayed from one captured hipGraph per
+        # wave type (captured on the first warmup call).
+        cases, wall, _, roots, _ = run_path("float32", "adaptive", graph=True)
+        for c in cases:
+            c.close()
+        result["graph"] = {"value": roots * args.steps / wall, "unit": "roots/s",
+                           "ms_per_step": wall / args.steps * 1e3, "valid_roots": roots}
+    if not args.no_e2e:
+        result["end_to_end"] = end_to_end_leg(wl, args.steps)
+        log(f"end to end: {result['end_to_end']['value']:.3g} roots/s (Solver calls on host arrays)")
     if not args.no_adjoint:
         result["fwd_adjoint"] = gradient_leg(torch, modisp_amd, workloads, args.steps, args.warmup)
         log(f"fwd+adjoint: {result['fwd_adjoint']['ms_per_step']:.2f} ms/step, "

@@ -216,6 +216,21 @@ int md_rocm_love_phase_kernel_report(int L, int single_precision, md_rocm_kernel
 int md_rocm_continuity_screen(const double *periods, int T, const void *phase, int phase_is_f32, int B, int K,
                               const uint32_t *prediction, int bilateral, uint8_t *flags, int *suspect);
 
+/* Per-cell result masks of a solve, host-side (reference DispersionBatch,
+ * python/modisp/__init__.py:100-123, computes them with numpy): velocity
+ * [B, modes, T] (float32 if velocity_is_f32, else float64), trace flags and
+ * root counts [B, T], group velocity and its flags [B, modes, T] (both NULL
+ * when group_mask is NULL). Writes status [B, modes, T] (0 valid, 1 not found,
+ * 2 unresolved, 3 rejected), mode_mask (finite), close_root_mask (a gap to a
+ * neighbouring mode at most close_root_rtol times the upper velocity, in the
+ * velocities' precision), close_root_risk [B, T] and group_mask (finite group
+ * velocity with zero flags), each 0 or 1 per byte. Models are spread over
+ * host threads. Returns 0, or 2 on a null pointer or bad size. */
+int md_rocm_result_masks(const void *velocity, int velocity_is_f32, const uint32_t *flags, const int32_t *counts,
+                         const double *group, const uint32_t *group_flags, int B, int modes, int T,
+                         double close_root_rtol, uint8_t *status, uint8_t *mode_mask, uint8_t *close_root_mask,
+                         uint8_t *close_root_risk, uint8_t *group_mask);
+
 /* Device helpers (reference cpp/cuda/qe.cu:1140-1142, bindings.cpp:354-357). */
 int md_rocm_available(int device); /* 1 if the device exists and is gfx950-capable */
 int md_rocm_set_device(int device);

@@ -132,6 +132,42 @@ def continuity_screen(periods, phase, prediction=None, bilateral=True):
     return out, bool(suspect.value)
 
 
+def result_masks(velocity, flags, counts, close_root_rtol, group=None, group_flags=None):
+    """Per-cell masks of a solve (reference DispersionBatch,
+    python/modisp/__init__.py:100-123): status [B, modes, T] (uint8: 0 valid,
+    1 not found, 2 unresolved, 3 rejected), mode_mask, close_root_mask,
+    close_root_risk [B, T] and, with group velocities, the group mask (finite
+    with zero flags; else None). One threaded host pass; the same values as the
+    reference's numpy expressions."""
+    v = np.asarray(velocity)
+    f32 = v.dtype == np.float32
+    v = np.ascontiguousarray(v, dtype=np.float32 if f32 else np.float64)
+    B, M, T = v.shape
+    fl = np.ascontiguousarray(flags, dtype=np.uint32)
+    ct = np.ascontiguousarray(counts, dtype=np.int32)
+    if fl.shape != (B, T) or ct.shape != (B, T):
+        raise ValueError("flags and counts must have shape [B,T]")
+    g = gf = gm = None
+    if group is not None:
+        g = np.ascontiguousarray(group, dtype=np.float64)
+        gf = np.ascontiguousarray(group_flags, dtype=np.uint32)
+        if g.shape != v.shape or gf.shape != v.shape:
+            raise ValueError("group velocity and flags must have the velocity's shape")
+        gm = np.empty(v.shape, dtype=bool)
+    status = np.empty(v.shape, dtype=np.uint8)
+    mode_mask = np.empty(v.shape, dtype=bool)
+    close = np.empty(v.shape, dtype=bool)
+    risk = np.empty((B, T), dtype=bool)
+    u8 = ctypes.POINTER(ctypes.c_uint8)
+    code = _native.lib().md_rocm_result_masks(
+        v.ctypes.data_as(ctypes.c_void_p), int(f32), _native.ptr(fl, _native._u32p), _native.ptr(ct, _native._i32p),
+        _native.ptr(g), _native.ptr(gf, _native._u32p), B, M, T, float(close_root_rtol), _native.ptr(status, u8),
+        _native.ptr(mode_mask, u8), _native.ptr(close, u8), _native.ptr(risk, u8), _native.ptr(gm, u8))
+    if code:
+        raise ValueError("invalid result mask arguments")
+    return status, mode_mask, close, risk, gm
+
+
 def _rows(a, name):
     a = np.ascontiguousarray(a, dtype=np.float64)
     if a.ndim != 2:

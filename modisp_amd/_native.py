@@ -49,7 +49,7 @@ EXPORTS = ("md_rocm_default_config", "md_rocm_create", "md_rocm_free", "md_rocm_
            "md_rocm_available", "md_rocm_set_device", "md_rocm_memory_info", "md_rocm_device_name",
            "md_rocm_last_error", "md_rocm_probe_grid", "md_rocm_last_stats", "md_rocm_adjoint",
            "md_rocm_set_stream", "md_rocm_autotune", "md_rocm_qe_kernel_report",
-           "md_rocm_love_phase_kernel_report", "md_rocm_continuity_screen")
+           "md_rocm_love_phase_kernel_report", "md_rocm_continuity_screen", "md_rocm_result_masks")
 
 
 def lib():
@@ -124,6 +124,10 @@ def lib():
     L.md_rocm_continuity_screen.argtypes = [_dp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32p,
                                             ctypes.c_int, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_int)]
     L.md_rocm_continuity_screen.restype = ctypes.c_int
+    _u8p = ctypes.POINTER(ctypes.c_uint8)
+    L.md_rocm_result_masks.argtypes = [_vp, ctypes.c_int, _u32p, _i32p, _dp, _u32p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_double, _u8p, _u8p, _u8p, _u8p, _u8p]
+    L.md_rocm_result_masks.restype = ctypes.c_int
     L.md_rocm_last_error.argtypes = []
     L.md_rocm_last_error.restype = ctypes.c_char_p
     _lib = L

@@ -28,7 +28,7 @@ from enum import IntEnum, IntFlag
 import numpy as np
 
 from . import QeSolver, BackendUnavailableError, available, bounds as _auto_bounds, set_device, STAT_NAMES
-from . import continuity_screen
+from . import continuity_screen, result_masks
 
 TRACE_UNRESOLVED_BRACKET = 1
 TRACE_RELOCATED = 2
@@ -117,8 +117,9 @@ class DispersionBatch:
         self.close_root_rtol = float(close_root_rtol)
         self.group_velocity = group_velocity
         self.group_velocity_flags = group_velocity_flags
-        self.group_velocity_mask = (None if group_velocity is None
-                                    else np.isfinite(group_velocity) & (group_velocity_flags == 0))
+        (self.status, self.mode_mask, self.close_root_mask, self.close_root_risk,
+         self.group_velocity_mask) = result_masks(velocity, flags, counts, close_root_rtol, group_velocity,
+                                                  group_velocity_flags)
         self.group_velocity_valid = self.group_velocity_mask
         self.prediction_flags = prediction_flags
         self.continuity_flags = continuity_flags
@@ -131,15 +132,7 @@ class DispersionBatch:
         self.mode_index_uncertain = self.approximate_search.copy()
         if continuity_unresolved:
             self.mode_index_uncertain |= (continuity_flags & ContinuityFlag.UNRESOLVED) != 0
-        self.close_root_mask = np.zeros(velocity.shape, dtype=bool)
-        if nmodes > 1:
-            gaps = np.diff(velocity, axis=1)
-            close = np.isfinite(gaps) & (gaps <= self.close_root_rtol * velocity[:, 1:, :])
-            self.close_root_mask[:, :-1, :] |= close
-            self.close_root_mask[:, 1:, :] |= close
-        self.close_root_risk = ((flags & TRACE_CLOSE_ROOT_RISK) != 0) | np.any(self.close_root_mask, axis=1)
         self.potentially_ambiguous = self.mode_index_uncertain | self.close_root_risk
-        self.mode_mask = np.isfinite(velocity)
         self.trace_flags = flags
         self.numerical_flags = flags & np.uint32(0xFFFFFFFF ^ (TRACE_RANGE_TRUNCATED | TRACE_AUTO_RELAXED))
         self.search_level = levels
@@ -147,12 +140,6 @@ class DispersionBatch:
         self.search_bounds = bounds.copy()
         self.backend_used = backend
         self.backend_reason = reason
-        unresolved = (flags[:, None, :] & (TRACE_SEARCH_UNCONVERGED | TRACE_UNRESOLVED_BRACKET)) != 0
-        self.status = np.where(self.mode_mask, SolveStatus.VALID,
-                               np.where(unresolved, SolveStatus.UNRESOLVED, SolveStatus.NOT_FOUND)).astype(np.uint8)
-        rejected = (np.isnan(velocity) & ((flags[:, None, :] & TRACE_F_REJECTED) != 0) &
-                    (np.arange(nmodes)[None, :, None] < counts[:, None, :]))
-        self.status[rejected] = SolveStatus.REJECTED
         self.mode_convention = ("ascending recovered roots; close-cluster omissions may shift indices"
                                 if np.any(self.approximate_search) else "ascending roots in the reported search interval")
 
@@ -395,12 +382,17 @@ class Solver:
             self._engine_key = key
         else:
             capacity = self._engine_key[0]
-        velocity = np.empty((B, nmodes, per.size), dtype=self.precision)
-        flags = np.empty((B, per.size), dtype=np.uint32)
-        levels = np.empty((B, per.size), dtype=np.int32)
-        counts = np.empty_like(levels)
-        group = np.empty(velocity.shape, dtype=np.float64) if self.group_velocity else None
-        gflags = np.empty(velocity.shape, dtype=np.uint32) if self.group_velocity else None
+        # One chunk: the engine's output arrays become the result as they are.
+        single = capacity >= B
+        if not single:
+            velocity = np.empty((B, nmodes, per.size), dtype=self.precision)
+            flags = np.empty((B, per.size), dtype=np.uint32)
+            levels = np.empty((B, per.size), dtype=np.int32)
+            counts = np.empty_like(levels)
+        group = gflags = None
+        if self.group_velocity and not single:
+            group = np.empty(velocity.shape, dtype=np.float64)
+            gflags = np.empty(velocity.shape, dtype=np.uint32)
         prediction = (np.full((B, per.size), int(PredictionFlag.BYPASSED), dtype=np.uint32)
                       if self.frequency_strategy == "anchor_continuation" else None)
         stats = {k: 0 for k in STAT_NAMES}
@@ -411,10 +403,15 @@ class Solver:
             out = self._engine.run(h[start:end], a[start:end], b[start:end], r[start:end], chunk_periods,
                                    bnd[start:end], self.c_tol_rel, self.f_accept, wave_code, self.cmin is None,
                                    self.profile)
-            velocity[start:end], flags[start:end], part, levels[start:end], counts[start:end], extras = out
-            if self.group_velocity:
-                group[start:end] = extras["group_velocity"]
-                gflags[start:end] = extras["group_velocity_flags"]
+            if single:
+                velocity, flags, part, levels, counts, extras = out
+                if self.group_velocity:
+                    group, gflags = extras["group_velocity"], extras["group_velocity_flags"]
+            else:
+                velocity[start:end], flags[start:end], part, levels[start:end], counts[start:end], extras = out
+                if self.group_velocity:
+                    group[start:end] = extras["group_velocity"]
+                    gflags[start:end] = extras["group_velocity_flags"]
             if continuation:
                 chunk_prediction = extras["prediction_flags"]
                 prediction[start:end] = chunk_prediction

@@ -9,7 +9,7 @@ HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
 LIBRARY = HERE / "libmodisp_rocm.so"
 SOURCES = [HERE / "csrc" / "kernels.hip", HERE / "csrc" / "adjoint.hip", HERE / "csrc" / "solver.cpp",
-           HERE / "csrc" / "bounds.cpp"]
+           HERE / "csrc" / "bounds.cpp", HERE / "csrc" / "results.cpp"]
 HEADERS = [HERE / "csrc" / "secular.hpp", HERE / "csrc" / "pipeline.hpp", ROOT / "include" / "modisp_rocm.h"]
 
 

@@ -31,6 +31,7 @@
 // and spreads their roots over its lanes; the weighted sums over each
 // model's roots, sum_roots w dc/dp, accumulate in LDS and are written once.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 #include "pipeline.hpp"
 #include "secular.hpp"
@@ -322,6 +323,21 @@ __global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs a) {
     }
     __syncthreads();
   }
+}
+
+// FP64 -> FP32 velocities for callers that take float32 results: one
+// grid-stride pass on the device, so half the bytes cross to the host.
+__global__ __launch_bounds__(256) void narrow_kernel(const double *in, float *out, size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = static_cast<float>(in[i]);
+}
+
+hipError_t launch_narrow(const double *in, float *out, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(narrow_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_adjoint(const AdjointArgs &a, int blocks, hipStream_t s) {

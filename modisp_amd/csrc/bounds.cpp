@@ -6,7 +6,10 @@
 // solid layer from the Rayleigh cubic, a Scholte speed when the model has a
 // water top, or the shear speed for Love; upper = min(requested or Vs of the
 // half-space, the largest double below Vs of the half-space).
+#include <algorithm>
 #include <cmath>
+#include <thread>
+#include <vector>
 
 #include "../../include/modisp_rocm.h"
 
@@ -65,11 +68,24 @@ double automatic_lower(const double *h, const double *vp, const double *vs, cons
 extern "C" int md_rocm_bounds(const double *thickness, const double *vp, const double *vs, const double *rho, int B,
                               int L, int wave, int auto_low, double low, int auto_high, double high, double *out) {
   if (!thickness || !vp || !vs || !rho || !out || B < 1 || L < 1) return 2;
-  for (int i = 0; i < B; ++i) {
-    const long long p = static_cast<long long>(i) * L;
-    out[2 * i] = auto_low ? automatic_lower(thickness + p, vp + p, vs + p, rho + p, L, wave) : low;
-    const double cap = std::nextafter(vs[p + L - 1], 0.);
-    out[2 * i + 1] = std::fmin(auto_high ? vs[p + L - 1] : high, cap);
+  auto window = [&](int b0, int b1) {
+    for (int i = b0; i < b1; ++i) {
+      const long long p = static_cast<long long>(i) * L;
+      out[2 * i] = auto_low ? automatic_lower(thickness + p, vp + p, vs + p, rho + p, L, wave) : low;
+      const double cap = std::nextafter(vs[p + L - 1], 0.);
+      out[2 * i + 1] = std::fmin(auto_high ? vs[p + L - 1] : high, cap);
+    }
+  };
+  // Models are independent: large batches are split over host threads.
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nthreads = static_cast<int>(std::min<long long>({16, hw, std::max(1LL, 1LL * B * L / 16384)}));
+  if (nthreads <= 1) {
+    window(0, B);
+    return 0;
   }
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nthreads; ++t)
+    pool.emplace_back(window, static_cast<int>(1LL * B * t / nthreads), static_cast<int>(1LL * B * (t + 1) / nthreads));
+  for (std::thread &th : pool) th.join();
   return 0;
 }

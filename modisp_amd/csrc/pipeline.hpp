@@ -317,6 +317,8 @@ struct AdjointArgs {
   int group;                        // models a block takes at once (LDS: group * 4 * L doubles)
 };
 hipError_t launch_adjoint(const AdjointArgs &a, int blocks, hipStream_t s);
+// out[i] = float(in[i]) (round to nearest even, NaN kept) for n values.
+hipError_t launch_narrow(const double *in, float *out, size_t n, hipStream_t s);
 // Compiled resources of the kernel that scans a model of L rows (which = 0)
 // or of the Love phase search (which = 1), for the plugin's kernel reports.
 struct KernelReport {

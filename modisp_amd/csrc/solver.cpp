@@ -74,7 +74,7 @@ struct md_rocm_solver {
   DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats, overflow, phase, tape;
   DevBuf c_anchors, c_targets, c_list, c_roots, c_slopes, c_fallback, c_count, c_pred;
   DevBuf in_h, in_vp, in_vs, in_rho, in_per, in_bounds;
-  DevBuf out_v, out_flags, out_levels, out_counts, out_group, out_gflags;
+  DevBuf out_v, out_flags, out_levels, out_counts, out_group, out_gflags, out_v32;
   double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<hipEvent_t> marks;  // kStageMarks per chunk of the last run
   int marked_chunks = 0;
@@ -92,7 +92,7 @@ struct md_rocm_solver {
                       &overflow, &phase, &tape, &c_anchors, &c_targets, &c_list, &c_roots, &c_slopes,
                       &c_fallback, &c_count, &c_pred,
                       &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &out_v, &out_flags, &out_levels,
-                      &out_counts, &out_group, &out_gflags, &table, &ntab})
+                      &out_counts, &out_group, &out_gflags, &out_v32, &table, &ntab})
       b->release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -704,13 +704,13 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
                                         want_group ? s->out_gflags.as<uint32_t>() : nullptr)) {
     return e;
   }
-  std::vector<double> vtmp;
-  double *vdst = static_cast<double *>(velocity);
-  if (velocity_is_f32) {
-    vtmp.resize(ncell);
-    vdst = vtmp.data();
+  if (velocity_is_f32) {  // narrowed on the device: half the bytes cross to the host
+    if (int e = s->out_v32.ensure(ncell * sizeof(float))) return e;
+    MD_HIP(mdr::launch_narrow(s->out_v.as<double>(), s->out_v32.as<float>(), ncell, st));
+    MD_HIP(hipMemcpyAsync(velocity, s->out_v32.p, ncell * sizeof(float), hipMemcpyDeviceToHost, st));
+  } else {
+    MD_HIP(hipMemcpyAsync(velocity, s->out_v.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
   }
-  MD_HIP(hipMemcpyAsync(vdst, s->out_v.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
   MD_HIP(hipMemcpyAsync(flags, s->out_flags.p, ntr * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   MD_HIP(hipMemcpyAsync(levels, s->out_levels.p, ntr * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   MD_HIP(hipMemcpyAsync(counts, s->out_counts.p, ntr * sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -719,10 +719,6 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
     MD_HIP(hipMemcpyAsync(group_flags, s->out_gflags.p, ncell * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   }
   if (int e = md_rocm_synchronize(s)) return e;
-  if (velocity_is_f32) {
-    float *out = static_cast<float *>(velocity);
-    for (size_t i = 0; i < ncell; ++i) out[i] = static_cast<float>(vtmp[i]);
-  }
   if (prediction_flags) {
     if (s->cfg.continuation) {
       MD_HIP(hipMemcpy(prediction_flags, s->c_pred.p, ntr * sizeof(uint32_t), hipMemcpyDeviceToHost));

@@ -183,3 +183,90 @@ def test_kernel_report_has_the_reference_keys(gpu):
     assert deep["table_capacity_per_model"] == 256 and deep["search_block_threads"] == 64
     with pytest.raises(ValueError):
         gpu.qe_kernel_report(0)
+
+
+def _numpy_masks(velocity, flags, counts, rtol, group=None, gflags=None):
+    """The reference DispersionBatch's numpy expressions for the per-cell masks
+    (python/modisp/__init__.py:100-123)."""
+    nmodes = velocity.shape[1]
+    close_mask = np.zeros(velocity.shape, dtype=bool)
+    if nmodes > 1:
+        gaps = np.diff(velocity, axis=1)
+        close = np.isfinite(gaps) & (gaps <= rtol * velocity[:, 1:, :])
+        close_mask[:, :-1, :] |= close
+        close_mask[:, 1:, :] |= close
+    risk = ((flags & api.TRACE_CLOSE_ROOT_RISK) != 0) | np.any(close_mask, axis=1)
+    mode_mask = np.isfinite(velocity)
+    status = np.where(mode_mask, api.SolveStatus.VALID,
+                      np.where((flags[:, None, :] & (api.TRACE_SEARCH_UNCONVERGED | api.TRACE_UNRESOLVED_BRACKET)) != 0,
+                               api.SolveStatus.UNRESOLVED, api.SolveStatus.NOT_FOUND)).astype(np.uint8)
+    rejected = (np.isnan(velocity) & ((flags[:, None, :] & api.TRACE_F_REJECTED) != 0) &
+                (np.arange(nmodes)[None, :, None] < counts[:, None, :]))
+    status[rejected] = api.SolveStatus.REJECTED
+    gmask = None if group is None else np.isfinite(group) & (gflags == 0)
+    return status, mode_mask, close_mask, risk, gmask
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("modes", [1, 5])
+@pytest.mark.parametrize("rtol", [.005, .007])
+def test_result_masks_equal_the_reference_numpy_expressions(dtype, modes, rtol):
+    """The native one-pass mask assembly (md_rocm_result_masks) against the
+    reference's numpy expressions on cells with missing roots, gaps at and
+    around the close-root threshold (in the velocities' own precision),
+    rejected / unresolved / close-risk traces and flagged group velocities.
+    With FP32 velocities and rtol = 0.007 some threshold gaps get the opposite
+    verdict in FP64 arithmetic; with rtol = 0.005 none can (f32(0.005) v
+    rounds to the FP64 threshold at every representable gap in [2, 4.5))."""
+    rng = np.random.default_rng(modes)
+    B, T = 700, 37
+    v = np.sort(rng.uniform(2., 4.5, (B, modes, T)), axis=1).astype(dtype)
+    split = 0
+    if modes > 1:
+        # Gaps within a few ulp of the threshold, and a share of them where the
+        # verdict differs between the array's precision and FP64 (the reference
+        # evaluates in the array's precision): such pairs are rare, so they are
+        # drawn from a pool found by search over many upper velocities.
+        up = rng.uniform(2., 4.5, 400_000).astype(dtype)
+        cand = [(up * (1 - rtol)).astype(dtype)]
+        for _ in range(4):
+            cand.append(np.nextafter(cand[-1], dtype(np.inf)))
+            cand.insert(0, np.nextafter(cand[0], dtype(0)))
+        cand = np.stack(cand)
+        own = (up - cand) <= dtype(rtol) * up
+        wide = (up.astype(np.float64) - cand.astype(np.float64)) <= rtol * up.astype(np.float64)
+        i, j = np.nonzero(own != wide)
+        pool_up, pool_low = up[j], cand[i, j]
+        near_up, near_low = up[:B * T], cand[rng.integers(0, 9, B * T), np.arange(B * T)]
+        assert (pool_up.size > 0) == (dtype == np.float32 and rtol == .007), pool_up.size
+        pick = rng.random(B * T) < .5
+        take = rng.integers(0, max(1, pool_up.size), B * T)
+        hi = np.where(pick & (pool_up.size > 0), pool_up[take] if pool_up.size else near_up, near_up)
+        lo = np.where(pick & (pool_up.size > 0), pool_low[take] if pool_up.size else near_low, near_low)
+        k = rng.random((B, T)) < .3
+        v[:, 2, :] = np.where(k, hi.reshape(B, T), v[:, 2, :])
+        v[:, 1, :] = np.where(k, lo.reshape(B, T), v[:, 1, :])
+        g = v[:, 2, :] - v[:, 1, :]
+        split = int((k & ((g <= dtype(rtol) * v[:, 2, :]) !=
+                          (v[:, 2, :].astype(np.float64) - v[:, 1, :].astype(np.float64)
+                           <= rtol * v[:, 2, :].astype(np.float64)))).sum())
+    v[rng.random(v.shape) < .15] = np.nan
+    flags = rng.choice(np.array([0, 0, 0, api.TRACE_F_REJECTED, api.TRACE_SEARCH_UNCONVERGED,
+                                 api.TRACE_UNRESOLVED_BRACKET, api.TRACE_CLOSE_ROOT_RISK,
+                                 api.TRACE_F_REJECTED | api.TRACE_UNRESOLVED_BRACKET], dtype=np.uint32), (B, T))
+    counts = rng.integers(-1, modes + 2, (B, T)).astype(np.int32)
+    group = np.where(rng.random(v.shape) < .1, np.nan, v.astype(np.float64) * .9)
+    gflags = np.where(rng.random(v.shape) < .1, 2, 0).astype(np.uint32)
+    for g, gf in ((group, gflags), (None, None)):
+        got = modisp_amd.result_masks(v, flags, counts, rtol, g, gf)
+        want = _numpy_masks(v, flags, counts, rtol, g, gf)
+        for a, b, name in zip(got, want, ("status", "mode_mask", "close_root_mask", "close_root_risk", "group")):
+            if b is None:
+                assert a is None, name
+                continue
+            assert a.dtype == b.dtype and a.shape == b.shape, name
+            np.testing.assert_array_equal(a, b, err_msg=name)
+    if modes > 1:
+        assert want[2].any() and not want[2].all()      # the threshold cases are exercised
+        assert split > 100 or not (dtype == np.float32 and rtol == .007)  # FP32 cells FP64 would flip
+    assert (want[0] == api.SolveStatus.REJECTED).any() and (want[0] == api.SolveStatus.UNRESOLVED).any()
