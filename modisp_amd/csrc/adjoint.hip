@@ -38,65 +38,6 @@
 
 namespace mdr {
 
-// One-direction forward-mode dual numbers (value, d/dk) for the forward
-// sweep, which needs dF/dk only.
-struct Dual1 {
-  double v, a;
-  __device__ Dual1() : v(0.), a(0.) {}
-  __device__ Dual1(double x) : v(x), a(0.) {}
-  __device__ Dual1(double x, double d) : v(x), a(d) {}
-};
-__device__ inline Dual1 operator+(Dual1 x, Dual1 y) { return {x.v + y.v, x.a + y.a}; }
-__device__ inline Dual1 operator-(Dual1 x, Dual1 y) { return {x.v - y.v, x.a - y.a}; }
-__device__ inline Dual1 operator-(Dual1 x) { return {-x.v, -x.a}; }
-__device__ inline Dual1 operator*(Dual1 x, Dual1 y) { return {x.v * y.v, fma(x.a, y.v, x.v * y.a)}; }
-__device__ inline Dual1 operator*(Dual1 x, double s) { return {x.v * s, x.a * s}; }
-__device__ inline Dual1 &operator*=(Dual1 &x, double s) { x = x * s; return x; }
-__device__ inline bool operator<(Dual1 x, Dual1 y) { return x.v < y.v; }
-__device__ inline bool operator==(Dual1 x, Dual1 y) { return x.v == y.v; }
-__device__ inline Dual1 m_sqrt(Dual1 x) {
-  const double s = sqrt(x.v);
-  return {s, x.a * (0.5 / s)};
-}
-// r = sqrt(r2) and 1/r with their k-derivatives from one reciprocal square
-// root: dr = dr2 / (2 r), d(1/r) = -dr2 / (2 r^3).
-__device__ inline void root_pair(Dual1 r2, Dual1 *r, Dual1 *ir) {
-  double s, is;
-  root_pair(r2.v, &s, &is);
-  const double h = 0.5 * is * r2.a;
-  *r = Dual1(s, h);
-  *ir = Dual1(is, -h * is * is);
-}
-__device__ inline void m_sincos(Dual1 x, Dual1 *s, Dual1 *c) {
-  double sn, cs;
-  m_sincos(x.v, &sn, &cs);
-  *s = Dual1(sn, cs * x.a);
-  *c = Dual1(cs, -sn * x.a);
-}
-__device__ inline Dual1 m_exp(Dual1 x) {
-  const double e = exp(x.v);
-  return {e, e * x.a};
-}
-__device__ inline Dual1 m_rcp(Dual1 x) {
-  const double r = 1. / x.v;
-  return {r, -x.a * r * r};
-}
-__device__ inline Dual1 m_fabs(Dual1 x) { return x.v < 0. ? -x : x; }
-__device__ inline Dual1 m_fma(Dual1 x, Dual1 y, Dual1 z) { return x * y + z; }
-template <int N>
-__device__ inline Dual1 rescale(Dual1 *e) {
-  double m = 0.;
-#pragma unroll
-  for (int i = 0; i < N; ++i) m = fmax(m, fabs(e[i].v));
-  if (m > 1.0e-40) {
-    const double inv = 1. / m;
-#pragma unroll
-    for (int i = 0; i < N; ++i) e[i] *= inv;
-    return Dual1(m);
-  }
-  return Dual1(1.);
-}
-
 __device__ inline PrepRow adj_row(const AdjointArgs &a, int m, int j) {
   const size_t i = static_cast<size_t>(m) * a.L + j;
   return make_prep_row(a.h[i], a.vp[i], a.vs[i], a.rho[i]);

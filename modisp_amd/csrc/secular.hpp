@@ -511,44 +511,188 @@ MDR_HD Dual2 rescale(Dual2 *e) {
 }  // namespace mdr
 
 // ---------------------------------------------------------------------------
+// Wave factor for derivatives at a root (dual numbers). The values are those
+// of wave_factor; the derivatives differ in two ways that matter only near a
+// body-wave speed (r -> 0), both exact at a root:
+//  * the attenuation e^-p of an evanescent layer is held constant: it
+//    multiplies every entry of the layer's matrix, so it is a positive factor
+//    of F, which vanishes at the root; its derivative grows like 1/r and the
+//    terms it adds would cancel only to rounding;
+//  * for |z| < 1/4, z = (kx^2 - k^2) h^2 (p^2 propagating, -p^2 evanescent),
+//    c, w / h and x / h come from the series C(z) = sum (-z)^n / (2n)! (cos
+//    sqrt z, cosh sqrt -z) and S(z) = sum (-z)^n / (2n+1)! (sin sqrt z / sqrt
+//    z, sinh sqrt -z / sqrt -z) to n = 7 (truncation below 1e-18), so no 1/r
+//    appears; the closed forms lose about eps / p^2 of the derivative of w.
+// ---------------------------------------------------------------------------
+namespace mdr {
+
+constexpr double kInvFact[16] = {1., 1., 0.5, 0.16666666666666666, 0.041666666666666664, 0.008333333333333333,
+                                 0.001388888888888889, 0.0001984126984126984, 2.48015873015873e-05,
+                                 2.7557319223985893e-06, 2.755731922398589e-07, 2.505210838544172e-08,
+                                 2.08767569878681e-09, 1.6059043836821613e-10, 1.1470745597729725e-11,
+                                 7.647163731819816e-13};  // 1 / n!
+
+template <class D>
+MDR_HD Wave<D> wave_factor_held(D k, D kx, D h) {
+  Wave<D> o;
+  const D r2s = (kx + k) * (kx - k);  // signed r^2: > 0 propagating
+  const D z = r2s * h * h;
+  if (std::fabs(z.v) < 0.25) {
+    const D u = -z;
+    D cs(kInvFact[14]), ss(kInvFact[15]);
+    for (int n = 6; n >= 0; --n) {
+      cs = cs * u + D(kInvFact[2 * n]);
+      ss = ss * u + D(kInvFact[2 * n + 1]);
+    }
+    const double e = k.v > kx.v ? std::exp(-std::sqrt(-z.v)) : 1.;  // e^-p, held
+    o.c = cs * e;
+    o.w = h * ss * e;
+    o.x = -(r2s * h * ss) * e;
+    o.att = D(e);
+  } else if (k.v < kx.v) {
+    D r, ir;
+    root_pair(r2s, &r, &ir);
+    const D p = r * h;
+    D sn, cn;
+    m_sincos(p, &sn, &cn);
+    o.c = cn;
+    o.w = sn * ir;
+    o.x = -(r * sn);
+    o.att = D(1.);
+  } else {
+    D r, ir;
+    root_pair(-r2s, &r, &ir);
+    const D p = r * h;
+    const double e = std::exp(-p.v), f = e * e, cv = (1. + f) * 0.5, sv = (1. - f) * 0.5;
+    const D dp = p - D(p.v);  // the derivative part of p
+    const D s = D(sv) + dp * cv;
+    o.c = D(cv) + dp * sv;
+    o.w = s * ir;
+    o.x = r * s;
+    o.att = D(e);
+  }
+  return o;
+}
+MDR_HD Wave<Dual2> wave_factor(Dual2 k, Dual2 kx, Dual2 h) { return wave_factor_held(k, kx, h); }
+
+// One-direction forward-mode dual numbers (value and one derivative): the
+// adjoint's forward sweep in k, and the gk-coefficients of its reverse step.
+struct Dual1 {
+  double v, a;
+  MDR_HD Dual1() : v(0.), a(0.) {}
+  MDR_HD Dual1(double x) : v(x), a(0.) {}
+  MDR_HD Dual1(double x, double d) : v(x), a(d) {}
+};
+MDR_HD Dual1 operator+(Dual1 x, Dual1 y) { return {x.v + y.v, x.a + y.a}; }
+MDR_HD Dual1 operator-(Dual1 x, Dual1 y) { return {x.v - y.v, x.a - y.a}; }
+MDR_HD Dual1 operator-(Dual1 x) { return {-x.v, -x.a}; }
+MDR_HD Dual1 operator*(Dual1 x, Dual1 y) { return {x.v * y.v, fma(x.a, y.v, x.v * y.a)}; }
+MDR_HD Dual1 operator*(Dual1 x, double s) { return {x.v * s, x.a * s}; }
+MDR_HD Dual1 &operator*=(Dual1 &x, double s) { x = x * s; return x; }
+MDR_HD bool operator<(Dual1 x, Dual1 y) { return x.v < y.v; }
+MDR_HD bool operator==(Dual1 x, Dual1 y) { return x.v == y.v; }
+MDR_HD Dual1 m_sqrt(Dual1 x) {
+  const double s = sqrt(x.v);
+  return {s, x.a * (0.5 / s)};
+}
+// r = sqrt(r2) and 1/r with their k-derivatives from one reciprocal square
+// root: dr = dr2 / (2 r), d(1/r) = -dr2 / (2 r^3).
+MDR_HD void root_pair(Dual1 r2, Dual1 *r, Dual1 *ir) {
+  double s, is;
+  root_pair(r2.v, &s, &is);
+  const double h = 0.5 * is * r2.a;
+  *r = Dual1(s, h);
+  *ir = Dual1(is, -h * is * is);
+}
+MDR_HD void m_sincos(Dual1 x, Dual1 *s, Dual1 *c) {
+  double sn, cs;
+  m_sincos(x.v, &sn, &cs);
+  *s = Dual1(sn, cs * x.a);
+  *c = Dual1(cs, -sn * x.a);
+}
+MDR_HD Dual1 m_exp(Dual1 x) {
+  const double e = exp(x.v);
+  return {e, e * x.a};
+}
+MDR_HD Dual1 m_rcp(Dual1 x) {
+  const double r = 1. / x.v;
+  return {r, -x.a * r * r};
+}
+MDR_HD Dual1 m_fabs(Dual1 x) { return x.v < 0. ? -x : x; }
+MDR_HD Dual1 m_fma(Dual1 x, Dual1 y, Dual1 z) { return x * y + z; }
+MDR_HD Wave<Dual1> wave_factor(Dual1 k, Dual1 kx, Dual1 h) { return wave_factor_held(k, kx, h); }
+template <int N>
+MDR_HD Dual1 rescale(Dual1 *e) {
+  double m = 0.;
+#pragma unroll
+  for (int i = 0; i < N; ++i) m = fmax(m, fabs(e[i].v));
+  if (m > 1.0e-40) {
+    const double inv = 1. / m;
+#pragma unroll
+    for (int i = 0; i < N; ++i) e[i] *= inv;
+    return Dual1(m);
+  }
+  return Dual1(1.);
+}
+
+
+}  // namespace mdr
+
+// ---------------------------------------------------------------------------
 // Reverse-mode derivative of one Rayleigh layer for the root adjoint: the
 // bilinear form S = l^T C e (C the layer's compound matrix at a fixed k) and
 // its derivatives with respect to the layer constants h, ka, kb, gk and rho
 // (rho also through 1/rho and 1/rho^2). S is linear in ten products of the
 // P and S wave factors (a0 = att_P att_S, cpcq = c_P c_S, ...); their
-// coefficients beta depend on gk, rho and the entry weights only, so the
-// gk and rho derivatives are sum_b beta_b' b (beta in duals of gk and rho),
-// and the h, ka, kb derivatives go through the adjoints of the wave factors.
+// coefficients beta depend on gk, rho and the entry weights only, so the gk
+// derivative is sum_b beta_b' b (beta in duals of gk), the h, ka, kb
+// derivatives go through the adjoints of the wave factors, and the rho
+// derivative follows from the entries' homogeneity in rho.
 // ---------------------------------------------------------------------------
 namespace mdr {
 
-// Wave factor with its derivatives along h and along kx (same branches as
-// wave_factor; at k == kx only the h derivative of w = h is kept).
+// Wave factor with its derivatives along h and along kx, as wave_factor_held
+// computes them: the attenuation held constant (datt = 0) and, for |z| < 1/4,
+// the series C(z), S(z) with their z-derivatives.
 struct WaveGrad {
   Wave<double> v, dh, dkx;
 };
 MDR_HD WaveGrad wave_factor_grad(double k, double kx, double h) {
   WaveGrad o;
+  const double r2s = (kx + k) * (kx - k), z = r2s * h * h;
+  if (std::fabs(z) < 0.25) {
+    const double u = -z;
+    double cs = kInvFact[14], ss = kInvFact[15], dcs = 0., dss = 0.;
+    for (int n = 6; n >= 0; --n) {  // Horner with the derivative in u
+      dcs = dcs * u + cs;
+      dss = dss * u + ss;
+      cs = cs * u + kInvFact[2 * n];
+      ss = ss * u + kInvFact[2 * n + 1];
+    }
+    const double cz = -dcs, sz = -dss;  // dC/dz, dS/dz
+    const double e = k > kx ? std::exp(-std::sqrt(-z)) : 1.;
+    const double zh = 2. * h * r2s, zk = 2. * kx * h * h;  // dz/dh, dz/dkx
+    o.v = {cs * e, h * ss * e, -r2s * h * ss * e, e};
+    o.dh = {cz * zh * e, (ss + h * sz * zh) * e, -r2s * (ss + h * sz * zh) * e, 0.};
+    o.dkx = {cz * zk * e, h * sz * zk * e, -(2. * kx * h * ss + r2s * h * sz * zk) * e, 0.};
+    return o;
+  }
   double r, ir;
   root_pair((k + kx) * std::fabs(k - kx), &r, &ir);
   const double p = r * h;
   if (k < kx) {  // p = r h, r^2 = kx^2 - k^2: dp/dh = r, dr/dkx = kx / r
-    double s, c;
-    m_sincos(p, &s, &c);
-    o.v = {c, s * ir, -r * s, 1.};
-    o.dh = {-s * r, c, -r * r * c, 0.};
+    double sn, cn;
+    m_sincos(p, &sn, &cn);
+    o.v = {cn, sn * ir, -r * sn, 1.};
+    o.dh = {-sn * r, cn, -r * r * cn, 0.};
     const double drk = kx * ir, dpk = h * drk;
-    o.dkx = {-s * dpk, (c * dpk - s * ir * drk) * ir, -(drk * s + r * c * dpk), 0.};
-  } else if (k == kx) {
-    o.v = {1., h, 0., 1.};
-    o.dh = {0., 1., 0., 0.};
-    o.dkx = {0., 0., 0., 0.};
-  } else {  // r^2 = k^2 - kx^2: dr/dkx = -kx / r; with f = e^-2p: c' = -f, s' = f, att' = -att
-    const double e = m_exp(-p), f = e * e, s = (1. - f) * 0.5;
-    o.v = {(1. + f) * 0.5, s * ir, r * s, e};
-    o.dh = {-f * r, f, r * r * f, -e * r};
+    o.dkx = {-sn * dpk, (cn * dpk - sn * ir * drk) * ir, -(drk * sn + r * cn * dpk), 0.};
+  } else {  // r^2 = k^2 - kx^2: dr/dkx = -kx / r; c = E cosh p, s = E sinh p with E = e^-p held
+    const double e = m_exp(-p), f = e * e, c = (1. + f) * 0.5, s = (1. - f) * 0.5;
+    o.v = {c, s * ir, r * s, e};
+    o.dh = {s * r, c, r * r * c, 0.};
     const double drk = -kx * ir, dpk = h * drk;
-    o.dkx = {-f * dpk, (f * dpk - s * ir * drk) * ir, drk * s + r * f * dpk, -e * dpk};
+    o.dkx = {s * dpk, (c * dpk - s * ir * drk) * ir, drk * s + r * c * dpk, 0.};
   }
   return o;
 }
@@ -594,43 +738,43 @@ MDR_HD LayerGrad rayleigh_layer_adjoint(double h, double ka, double kb, double g
   const double a0 = P.v.att * S.v.att, cpcq = P.v.c * S.v.c, cpy = P.v.c * S.v.w, cpz = P.v.c * S.v.x;
   const double cqw = S.v.c * P.v.w, cqx = S.v.c * P.v.x, xy = P.v.x * S.v.w, xz = P.v.x * S.v.x;
   const double wy = P.v.w * S.v.w, wz = P.v.w * S.v.x;
-  // their coefficients, in duals of (gk, rho)
-  const Dual2 g(gk, 1., 0.), rh(rho, 0., 1.), irh(irho, 0., -irho * irho), irh2(irho2, 0., -2. * irho2 * irho);
-  const Dual2 gam = g * k2, gm1 = gam - Dual2(1.), tw = gam + gm1, gg = gam * g, gm = gam * gm1, g2s = gm1 * gm1;
-  const Dual2 rr = rh * rh, ggr = gg * rh, g2r = g2s * rh;
+  // their coefficients, in duals of gk
+  const Dual1 g(gk, 1.), gam = g * k2, gm1 = gam - Dual1(1.), tw = gam + gm1, gg = gam * g, gm = gam * gm1;
+  const Dual1 g2s = gm1 * gm1;
+  const double rr = rho * rho;
   const double ap = w.A - 2. * w.K;  // K = a0 + 2 (cpcq - A)
-  const Dual2 Q = -(gm * (2. * ap) + tw * irh * w.I + irh2 * (2. * k2 * w.Lq) + gg * g2s * rr * (2. * w.O) +
-                    g * gm1 * tw * rh * w.J);
-  const Dual2 b_a0 = Q + Dual2(w.K);
-  const Dual2 b_cpcq = Dual2(ap + 2. * w.K + w.B) - Q;
-  const Dual2 b_cpy = irh * (k2 * w.D) + g2r * w.F + gm1 * w.H;
-  const Dual2 b_cqx = -(irh * w.D + ggr * w.F + g * w.H);
-  const Dual2 b_cpz = irh * w.C + ggr * w.E + g * w.G;
-  const Dual2 b_cqw = -(irh * (k2 * w.C) + g2r * w.E + gm1 * w.G);
-  const Dual2 b_xz = -(gg * ap + g * irh * w.I + irh2 * w.Lq + gg * gg * rr * w.O + gg * g * rh * w.J);
-  const Dual2 b_wy = -(g2s * (k2 * ap) + gm1 * irh * (k2 * w.I) + irh2 * (k4 * w.Lq) + g2s * g2s * rr * w.O +
-                       gm1 * g2r * w.J);
+  const Dual1 Q = -(gm * (2. * ap) + tw * (irho * w.I) + Dual1(irho2 * (2. * k2 * w.Lq)) +
+                    gg * g2s * (rr * 2. * w.O) + g * gm1 * tw * (rho * w.J));
+  const Dual1 b_a0 = Q + Dual1(w.K);
+  const Dual1 b_cpcq = Dual1(ap + 2. * w.K + w.B) - Q;
+  const Dual1 b_cpy = Dual1(irho * k2 * w.D) + g2s * (rho * w.F) + gm1 * w.H;
+  const Dual1 b_cqx = -(Dual1(irho * w.D) + gg * (rho * w.F) + g * w.H);
+  const Dual1 b_cpz = Dual1(irho * w.C) + gg * (rho * w.E) + g * w.G;
+  const Dual1 b_cqw = -(Dual1(irho * k2 * w.C) + g2s * (rho * w.E) + gm1 * w.G);
+  const Dual1 b_xz = -(gg * ap + g * (irho * w.I) + Dual1(irho2 * w.Lq) + gg * gg * (rr * w.O) + gg * g * (rho * w.J));
+  const Dual1 b_wy = -(g2s * (k2 * ap) + gm1 * (irho * k2 * w.I) + Dual1(irho2 * k4 * w.Lq) + g2s * g2s * (rr * w.O) +
+                       gm1 * g2s * (rho * w.J));
   const double b_wz = -w.M, b_xy = -w.N;
   LayerGrad d;
   d.gk = b_a0.a * a0 + b_cpcq.a * cpcq + b_cpy.a * cpy + b_cqx.a * cqx + b_cpz.a * cpz + b_cqw.a * cqw +
          b_xz.a * xz + b_wy.a * wy;
-  d.rho = b_a0.b * a0 + b_cpcq.b * cpcq + b_cpy.b * cpy + b_cqx.b * cqx + b_cpz.b * cpz + b_cqw.b * cqw +
-          b_xz.b * xz + b_wy.b * wy;
-  // adjoints of the wave factors
+  // adjoints of the wave factors (their attenuation is held: no att terms)
   const double pc = b_cpcq.v * S.v.c + b_cpy.v * S.v.w + b_cpz.v * S.v.x;
   const double pw = b_cqw.v * S.v.c + b_wy.v * S.v.w + b_wz * S.v.x;
   const double px = b_cqx.v * S.v.c + b_xy * S.v.w + b_xz.v * S.v.x;
-  const double pa = b_a0.v * S.v.att;
   const double sc = b_cpcq.v * P.v.c + b_cqw.v * P.v.w + b_cqx.v * P.v.x;
   const double sw = b_cpy.v * P.v.c + b_xy * P.v.x + b_wy.v * P.v.w;
   const double sx = b_cpz.v * P.v.c + b_xz.v * P.v.x + b_wz * P.v.w;
-  const double sa = b_a0.v * P.v.att;
-  d.ka = pc * P.dkx.c + pw * P.dkx.w + px * P.dkx.x + pa * P.dkx.att;
-  d.kb = sc * S.dkx.c + sw * S.dkx.w + sx * S.dkx.x + sa * S.dkx.att;
-  d.h = pc * P.dh.c + pw * P.dh.w + px * P.dh.x + pa * P.dh.att + sc * S.dh.c + sw * S.dh.w + sx * S.dh.x +
-        sa * S.dh.att;
+  d.ka = pc * P.dkx.c + pw * P.dkx.w + px * P.dkx.x;
+  d.kb = sc * S.dkx.c + sw * S.dkx.w + sx * S.dkx.x;
+  d.h = pc * P.dh.c + pw * P.dh.w + px * P.dh.x + sc * S.dh.c + sw * S.dh.w + sx * S.dh.x;
+  // The entries are homogeneous in rho (degree -1: D, I, C; -2: Lq; +1: E,
+  // F, J; +2: O; 0: the rest), so rho dS/drho = sum_X degree_X w_X X.
+  const Compound<double> q = compound_from_waves(P.v, S.v, gk, rho, irho, irho2, k2);
+  d.rho = irho * (2. * (w.O * q.O - w.Lq * q.Lq) + w.E * q.E + w.F * q.F + w.J * q.J - w.D * q.D - w.I * q.I -
+                  w.C * q.C);
   // l <- l^T C from the entries' values
-  compound_apply_transpose(compound_from_waves(P.v, S.v, gk, rho, irho, irho2, k2), l);
+  compound_apply_transpose(q, l);
   return d;
 }
 

@@ -240,3 +240,35 @@ def reference_result(h, vp, vs, rho, periods, nmodes, wave="rayleigh", cmin=None
     status[rejected] = 3
     return dict(phase_velocity=velocity, mode_mask=mask, status=status, trace_flags=flags,
                 search_level=out["search_level"], root_count=counts, search_bounds=b)
+
+
+# --- references for derivative checks --------------------------------------------------------
+def curve_group_velocity(h, vp, vs, rho, period, window, mode, wave):
+    """U = d(omega)/dk from phase roots at neighbouring frequencies (central
+    differences of k(omega) = omega / c, Richardson-extrapolated)."""
+    w0 = 2 * np.pi / period
+
+    def k_of(w):
+        c = solve(h, vp, vs, rho, [2 * np.pi / w], window, mode + 1, wave, tol=1e-14)["phase_velocity"]
+        return w / c[0, mode, 0]
+    d = 5e-4 * w0
+    d1 = (k_of(w0 + d) - k_of(w0 - d)) / (2 * d)
+    d2 = (k_of(w0 + 2 * d) - k_of(w0 - 2 * d)) / (4 * d)
+    return 3 / (4 * d1 - d2)
+
+
+def shear_speed_crossing(wave):
+    """A four-row model and the period at which its fundamental mode's phase
+    velocity sits 1e-9 below the second layer's shear speed (bisection in
+    log T): there r -> 0 in that layer."""
+    h = np.array([1.1, 2.3, 4.7, 0.])
+    vs = np.array([1.5, 2.2, 2.9, 3.8])
+    vp, rho = 1.8 * vs, .32 * 1.8 * vs + .77
+    win = bounds(h, vp, vs, rho, wave)
+    target = vs[1] * (1 - 1e-9)
+    lo, hi = np.log(.5), np.log(50.)
+    for _ in range(60):
+        mid = .5 * (lo + hi)
+        c = solve(h, vp, vs, rho, [np.exp(mid)], win, 1, wave, tol=1e-15)["phase_velocity"][0, 0, 0]
+        lo, hi = (mid, hi) if c < target else (lo, mid)
+    return h, vp, vs, rho, np.exp(.5 * (lo + hi)), win

@@ -5,11 +5,16 @@
 //    implicit group velocity (reference cpp/common/group_velocity.hpp:9-104)
 //    and its neighbour window (cpp/cpu/solver.cpp:96-103);
 //  * forward-mode (one-direction dual number) derivatives of F with respect
-//    to k, omega and every layer parameter. The reference has no analytic
+//    to k, omega and every layer parameter, with the positive per-layer
+//    attenuation and normalization factors held constant, so they equal the
+//    derivatives of F at a root (where F = 0) and stay well conditioned near
+//    a body-wave speed (wave_parts). The reference has no analytic
 //    derivative; these are this oracle's own, checked in tests against closed
-//    forms (half-space U=c, two-layer Love implicit derivative) and against
-//    Richardson-extrapolated central differences of the released evaluator.
-//    They pin the GPU's analytic group velocity and adjoint dc/dm.
+//    forms (half-space U=c, two-layer Love implicit derivative), against
+//    Richardson-extrapolated central differences of the released evaluator at
+//    roots, and against differences of the dispersion curve itself next to a
+//    layer's shear speed. They pin the GPU's analytic group velocity and
+//    adjoint dc/dm.
 #pragma once
 #include "md_oracle.hpp"
 
@@ -151,6 +156,53 @@ inline double dcos(double a) { return std::cos(a); }
 inline double dexp(double a) { return std::exp(a); }
 inline double scaled(double a, double s) { return a / s; }
 
+// One wave type's layer factors for derivatives at a root: c = cos p,
+// w = sin p / r, x = -r sin p when propagating (k < kx), and e^-p (cosh p,
+// sinh p / r, r sinh p) when evanescent, p = r h; pex = p when evanescent
+// (else 0) feeds the layer attenuation e^-(pex + sex). Two choices make the
+// derivatives exact at a root and well conditioned near r = 0:
+//  * the attenuation e^-p is held constant: it multiplies every entry of the
+//    layer's matrix, a positive factor of F, which vanishes at the root;
+//  * for |z| < 1/4, z = (kx^2 - k^2) h^2, c, w / h and x / h are the power
+//    series of cos sqrt z and sin sqrt z / sqrt z in z (cosh and sinh for
+//    z < 0) to the z^7 term, so no 1/r appears.
+// The values are those of the closed forms to rounding.
+inline void wave_parts(Dual k, Dual kx, Dual h, Dual &c, Dual &w, Dual &x, double &pex) {
+  static const double inv_fact[16] = {1., 1., 1. / 2., 1. / 6., 1. / 24., 1. / 120., 1. / 720., 1. / 5040.,
+                                      1. / 40320., 1. / 362880., 1. / 3628800., 1. / 39916800., 1. / 479001600.,
+                                      1. / 6227020800., 1. / 87178291200., 1. / 1307674368000.};
+  pex = 0.;
+  const Dual r2s = (kx + k) * (kx - k), z = r2s * h * h;
+  if (std::fabs(z.v) < .25) {
+    const Dual u = -z;
+    Dual cs(inv_fact[14]), ss(inv_fact[15]);
+    for (int n = 6; n >= 0; --n) {
+      cs = cs * u + Dual(inv_fact[2 * n]);
+      ss = ss * u + Dual(inv_fact[2 * n + 1]);
+    }
+    double e = 1.;
+    if (k.v > kx.v) { pex = std::sqrt(-z.v); e = std::exp(-pex); }
+    c = cs * Dual(e);
+    w = h * ss * Dual(e);
+    x = -(r2s * h * ss) * Dual(e);
+    return;
+  }
+  if (k.v < kx.v) {
+    const Dual r = dsqrt(r2s), p = r * h, sn = dsin(p);
+    c = dcos(p);
+    w = sn / r;
+    x = -r * sn;
+    return;
+  }
+  const Dual r = dsqrt(-r2s), p = r * h;
+  pex = p.v;
+  const double f = p.v < 16. ? std::exp(-2. * p.v) : 0., cv = (1. + f) * .5, sv = (1. - f) * .5;
+  const Dual sh(sv, cv * p.d);  // e^-p sinh p with e^-p held
+  c = Dual(cv, sv * p.d);
+  w = sh / r;
+  x = r * sh;
+}
+
 // Released Rayleigh recurrence on a generic scalar (solid or water top). The
 // per-layer max-abs normalization uses the values only: the positive scale
 // multiplies F and vanishes from any derivative ratio taken at a root.
@@ -172,30 +224,13 @@ T rayleigh_generic(const T *d, const T *a, const T *b, const T *rho, int L, T om
     t = b[j] / omega;
     gammk = T(2.) * t * t;
     gam = gammk * k2;
-    ra = radical(ka);
-    rb = radical(kb);
-    const T h = d[j], p = ra * h, q = rb * h;
-    T cosp(0.), w(0.), x(0.), cosq(0.), y(0.), z(0.), pex(0.), sex(0.);
-    if (val(k) < val(ka)) { const T s = dsin(p); cosp = dcos(p); w = s / ra; x = -ra * s; }
-    else if (val(k) == val(ka)) { cosp = T(1.); w = h; x = T(0.); }
-    else {
-      pex = p;
-      const T fac = val(p) < 16. ? dexp(T(-2.) * p) : T(0.);
-      cosp = (T(1.) + fac) * T(.5);
-      const T s = (T(1.) - fac) * T(.5);
-      w = s / ra; x = ra * s;
-    }
-    if (val(k) < val(kb)) { const T s = dsin(q); cosq = dcos(q); y = s / rb; z = -rb * s; }
-    else if (val(k) == val(kb)) { cosq = T(1.); y = h; z = T(0.); }
-    else {
-      sex = q;
-      const T fac = val(q) < 16. ? dexp(T(-2.) * q) : T(0.);
-      cosq = (T(1.) + fac) * T(.5);
-      const T s = (T(1.) - fac) * T(.5);
-      y = s / rb; z = rb * s;
-    }
-    const T exa = pex + sex;
-    const T a0 = val(exa) == 0. ? T(1.) : (val(exa) < 60. ? dexp(-exa) : T(0.));
+    const T h = d[j];
+    T cosp, w, x, cosq, y, z;
+    double pex, sex;
+    wave_parts(k, ka, h, cosp, w, x, pex);
+    wave_parts(k, kb, h, cosq, y, z, sex);
+    const double exa = pex + sex;
+    const T a0(exa == 0. ? 1. : (exa < 60. ? std::exp(-exa) : 0.));  // held, as each factor
     const T cpcq = cosp * cosq, cpy = cosp * y, cpz = cosp * z, cqw = cosq * w, cqx = cosq * x;
     const T xy = x * y, xz = x * z, wy = w * y, wz = w * z;
     const T rhol = rho[j];
@@ -239,15 +274,10 @@ T rayleigh_generic(const T *d, const T *a, const T *b, const T *rho, int L, T om
     for (int i = 0; i < 5; ++i) e[i] = scaled(ee[i], scale);
   }
   if (water) {
-    const T kf = omega / a[0], rf = radical(kf), p = rf * d[0];
-    T cp, w;
-    if (val(k) < val(kf)) { cp = dcos(p); w = dsin(p) / rf; }
-    else if (val(k) == val(kf)) { cp = T(1.); w = d[0]; }
-    else {
-      const T z = val(p) < 16. ? dexp(T(-2.) * p) : T(0.);
-      cp = (T(1.) + z) * T(.5);
-      w = (T(1.) - z) * T(.5) / rf;
-    }
+    const T kf = omega / a[0];
+    T cp, w, x;
+    double pex;
+    wave_parts(k, kf, d[0], cp, w, x, pex);
     return cp * e[0] - rho[0] * w * e[1];
   }
   return e[0];
@@ -261,16 +291,9 @@ T love_generic(const T *h, const T *vs, const T *rho, int L, T omega, T k) {
   T e2 = T(1.) / (vs[last] * vs[last]);
   for (int j = last - 1; j >= top; --j) {
     const T mu = rho[j] * vs[j] * vs[j], kj = omega / vs[j];
-    const T r = dsqrt((k + kj) * dfabs(k - kj)), q = r * h[j];
     T cq, y, z;
-    if (val(k) < val(kj)) { const T s = dsin(q); cq = dcos(q); y = s / r; z = -r * s; }
-    else if (val(k) == val(kj)) { cq = T(1.); y = h[j]; z = T(0.); }
-    else {
-      const T f = val(q) < 16. ? dexp(T(-2.) * q) : T(0.);
-      cq = (T(1.) + f) * T(.5);
-      y = (T(1.) - f) * T(.5) / r;
-      z = (T(1.) - f) * T(.5) * r;
-    }
+    double sex;
+    wave_parts(k, kj, h[j], cq, y, z, sex);
     const T u = e1 * cq + e2 * mu * z, v = e1 * y / mu + e2 * cq;
     double s = std::fmax(std::fabs(val(u)), std::fabs(val(v)));
     if (s < 1e-40) s = 1.;

@@ -458,7 +458,9 @@ double mdo_group_analytic(const double *h, const double *vp, const double *vs, c
   return group_velocity_analytic(h, vp, vs, rho, L, wave, two_pi() / period, c);
 }
 
-// dF/d(which) at (period, c); which as in secular_direction.
+// dF/d(which) at (period, c) with the positive attenuation and normalization
+// factors held constant (the derivative of F itself at a root); which as in
+// secular_direction.
 double mdo_secular_derivative(const double *h, const double *vp, const double *vs, const double *rho, int L, int wave,
                               double period, double c, int which) {
   return secular_direction(h, vp, vs, rho, L, wave, two_pi() / period, c, which).d;

@@ -6,7 +6,9 @@
 //  * central differences of S in long double (for gk and rho, whose
 //    derivatives lose digits to cancellation in any double evaluation).
 // Samples cover both wave factors propagating, P evanescent and S
-// propagating, both evanescent, and k exactly at the P wavenumber.
+// propagating, both evanescent, k exactly at the P wavenumber, and c within
+// 1e-7 of the shear speed (the small-z series of the wave factors; thin
+// layers at long periods reach it too).
 // Prints: worst relative difference reverse vs forward for (h, ka, kb), the
 // same for (gk, rho), worst relative difference of the carried row l^T C,
 // and the worst relative errors of reverse and of forward mode against the
@@ -58,8 +60,9 @@ int main() {
     const double vp = U(1.5, 8.), vs = vp * U(0.3, 0.7), rho = U(1.5, 3.5);
     const double h = std::exp(U(std::log(0.01), std::log(50.)));
     const double ka = omega / vp, kb = omega / vs, gk = 2. * vs * vs / (omega * omega);
-    const int regime = it % 4;
-    const double c = regime == 0 ? vp * U(1.01, 1.5) : regime == 1 ? U(vs * 1.001, vp * 0.999) : vs * U(0.5, 0.999);
+    const int regime = it % 5;
+    const double c = regime == 0 ? vp * U(1.01, 1.5) : regime == 1 ? U(vs * 1.001, vp * 0.999)
+                   : regime == 4 ? vs * (1. + 1e-7 * U(-1., 1.)) : vs * U(0.5, 0.999);
     const double k = regime == 3 ? ka : omega / c;
     const double irho = 1. / rho, irho2 = irho * irho, k2 = k * k;
     double l[5], e[5];
@@ -72,13 +75,18 @@ int main() {
                                s(irho, 4, -irho * irho), s(irho2, 4, -2. * irho2 * irho), Dual2(0.)};
     };
     const Dual2 kd(k), k2d(k2);
-    const Dual2 s1 = bilinear(mdr::rayleigh_compound(layer(0, 1), kd, k2d), w);
+    const mdr::Compound<Dual2> q1 = mdr::rayleigh_compound(layer(0, 1), kd, k2d);
+    const Dual2 s1 = bilinear(q1, w);
     const Dual2 s2 = bilinear(mdr::rayleigh_compound(layer(2, 3), kd, k2d), w);
     const Dual2 s3 = bilinear(mdr::rayleigh_compound(layer(4, -1), kd, k2d), w);
     const double fwd[5] = {s1.a, s1.b, s2.a, s2.b, s3.a};
+    // the carried row from the forward-mode entry values (the values the
+    // adjoint's forward sweep tapes)
     double lt[5] = {l[0], l[1], l[2], l[3], l[4]};
-    mdr::compound_apply_transpose(
-        mdr::rayleigh_compound(mdr::Layer<double>{h, ka, kb, gk, rho, irho, irho2, 0.}, k, k2), lt);
+    mdr::compound_apply_transpose(mdr::Compound<double>{q1.A.v, q1.B.v, q1.C.v, q1.D.v, q1.E.v, q1.F.v, q1.G.v,
+                                                        q1.H.v, q1.I.v, q1.J.v, q1.K.v, q1.Lq.v, q1.M.v, q1.N.v,
+                                                        q1.O.v, q1.t.v},
+                                  lt);
     double lr[5] = {l[0], l[1], l[2], l[3], l[4]};
     const mdr::LayerGrad d = mdr::rayleigh_layer_adjoint(h, ka, kb, gk, rho, irho, irho2, k, lr, e);
     const double rev[5] = {d.h, d.ka, d.kb, d.gk, d.rho};

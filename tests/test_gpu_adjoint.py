@@ -40,6 +40,10 @@ def _cases():
     yield "gradient-love", wl.thickness, wl.vp, wl.vs, wl.rho, wl.periods[::10], 3, 1
     h, vp, vs, rho = _water_models(4, np.random.default_rng(11))
     yield "water-rayleigh", h, vp, vs, rho, np.array([1., 4., 12.]), 3, 0
+    # a root 1e-9 below a layer's shear speed (r -> 0 in that layer)
+    for wave, name in ((0, "shear-crossing-rayleigh"), (1, "shear-crossing-love")):
+        h, vp, vs, rho, period, _ = oracle.shear_speed_crossing(wave)
+        yield name, h[None], vp[None], vs[None], rho[None], np.array([period]), 1, wave
 
 
 @pytest.mark.parametrize("case", list(_cases()), ids=lambda c: c[0])

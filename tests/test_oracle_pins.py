@@ -215,6 +215,21 @@ def test_secular_derivatives_match_richardson(wave):
                 assert abs(got - want) <= 1e-6 * abs(want) + 1e-10 * scale, (j, p, got, want)
 
 
+@pytest.mark.parametrize("wave", [0, 1], ids=["rayleigh", "love"])
+def test_group_velocity_next_to_a_shear_speed(wave):
+    """At a root within 1e-9 of a layer's shear speed the layer's radial
+    wavenumber r is ~1e-5 of k: the closed-form derivatives of sin(rh)/r and of
+    the evanescent attenuation lose up to 1e-5 of U there. The analytic group
+    velocity (attenuation held, small-z series) against differences of the
+    dispersion curve itself."""
+    h, vp, vs, rho, period, win = oracle.shear_speed_crossing(wave)
+    r = oracle.solve(h, vp, vs, rho, [period], win, 1, wave, tol=1e-13, group=2)
+    c, u = r["phase_velocity"][0, 0, 0], r["group_velocity"][0, 0, 0]
+    assert abs(vs[1] - c) <= 2e-9 * c
+    want = oracle.curve_group_velocity(h, vp, vs, rho, period, win, 0, wave)
+    assert abs(u - want) <= 1e-10 * want, (u, want)
+
+
 def test_phase_gradient_matches_root_finite_differences():
     h = np.array([1.4, 2.6, 1.1, 0.])
     vs = np.array([1.6, 2.3, 3.1, 4.0])
