@@ -1246,36 +1246,87 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(PipeArgs a) {
 
 // ---------------------------------------------------------------------------
 // Analytic group velocity U = dw/dk = -F_k / F_w at each refined root.
+//
+// GroupDual: two-direction dual numbers (d/dk, d/dw) of their own, so that the
+// group kernel's wave factor can take the narrow series region
+// (wave_factor_held<GroupDual, true>): lanes of a wave (different traces and
+// modes) then rarely straddle the series and closed-form branches.
 // ---------------------------------------------------------------------------
+struct GroupDual {
+  double v, a, b;
+  __device__ GroupDual() : v(0.), a(0.), b(0.) {}
+  __device__ GroupDual(double x) : v(x), a(0.), b(0.) {}
+  __device__ GroupDual(double x, double da, double db) : v(x), a(da), b(db) {}
+};
+using GD = GroupDual;
+__device__ __forceinline__ GD operator+(GD x, GD y) { return {x.v + y.v, x.a + y.a, x.b + y.b}; }
+__device__ __forceinline__ GD operator-(GD x, GD y) { return {x.v - y.v, x.a - y.a, x.b - y.b}; }
+__device__ __forceinline__ GD operator-(GD x) { return {-x.v, -x.a, -x.b}; }
+__device__ __forceinline__ GD operator*(GD x, GD y) { return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b}; }
+__device__ __forceinline__ GD operator*(GD x, double s) { return {x.v * s, x.a * s, x.b * s}; }
+__device__ __forceinline__ GD &operator*=(GD &x, double s) { x = x * s; return x; }
+__device__ __forceinline__ bool operator<(GD x, GD y) { return x.v < y.v; }
+__device__ __forceinline__ bool operator==(GD x, GD y) { return x.v == y.v; }
+__device__ __forceinline__ GD m_sqrt(GD x) {
+  const double r = sqrt(x.v), d = 0.5 / r;
+  return {r, x.a * d, x.b * d};
+}
+__device__ __forceinline__ void m_sincos(GD x, GD *s, GD *c) {
+  double sn, cs;
+  m_sincos(x.v, &sn, &cs);
+  *s = GD(sn, cs * x.a, cs * x.b);
+  *c = GD(cs, -sn * x.a, -sn * x.b);
+}
+__device__ __forceinline__ GD m_rcp(GD x) {
+  const double r = 1. / x.v, d = -r * r;
+  return {r, x.a * d, x.b * d};
+}
+__device__ __forceinline__ GD m_fabs(GD x) { return x.v < 0. ? -x : x; }
+__device__ __forceinline__ GD m_fma(GD x, GD y, GD z) { return x * y + z; }
+__device__ __forceinline__ Wave<GD> wave_factor(GD k, GD kx, GD h) { return wave_factor_held<GD, true>(k, kx, h); }
+template <int N>
+__device__ __forceinline__ GD rescale(GD *e) {  // the scale is held constant
+  double m = 0.;
+#pragma unroll
+  for (int i = 0; i < N; ++i) m = fmax(m, fabs(e[i].v));
+  if (m > 1.0e-40) {
+    const double inv = 1. / m;
+#pragma unroll
+    for (int i = 0; i < N; ++i) e[i] *= inv;
+    return GD(m);
+  }
+  return GD(1.);
+}
+
 __device__ __forceinline__ double group_at(const PrepRow *rows, int L, int top, int wave, double omega, double c) {
   const double iw = 1. / omega, iw2 = iw * iw;
-  const Dual2 k(omega / c, 1., 0.), w(omega, 0., 1.);
+  const GD k(omega / c, 1., 0.), w(omega, 0., 1.);
   auto layer = [&](const PrepRow &r) {
-    Layer<Dual2> l;
-    l.h = Dual2(r.h);
-    l.kb = Dual2(omega * r.ivs, 0., r.ivs);
+    Layer<GD> l;
+    l.h = GD(r.h);
+    l.kb = GD(omega * r.ivs, 0., r.ivs);
     if (wave == 0) {
-      l.ka = Dual2(omega * r.ivp, 0., r.ivp);
-      l.gk = Dual2(r.g2 * iw2, 0., -2. * r.g2 * iw2 * iw);
-      l.rho = Dual2(r.rho);
-      l.irho = Dual2(r.irho);
-      l.irho2 = Dual2(r.irho2);
+      l.ka = GD(omega * r.ivp, 0., r.ivp);
+      l.gk = GD(r.g2 * iw2, 0., -2. * r.g2 * iw2 * iw);
+      l.rho = GD(r.rho);
+      l.irho = GD(r.irho);
+      l.irho2 = GD(r.irho2);
     } else {
-      l.rho = Dual2(0.5 * r.rho * r.g2);
-      l.irho = Dual2(r.imu);
+      l.rho = GD(0.5 * r.rho * r.g2);
+      l.irho = GD(r.imu);
     }
     return l;
   };
-  Dual2 F;
+  GD F;
   if (wave == 0) {
-    const Dual2 k2 = k * k;
-    Dual2 e[5];
+    const GD k2 = k * k;
+    GD e[5];
     rayleigh_halfspace(layer(rows[L - 1]), k, k2, e);
     for (int j = L - 2; j >= top; --j) rayleigh_layer(layer(rows[j]), k, k2, e);
     F = top ? rayleigh_water_top(layer(rows[0]), k, e) : e[0];
   } else {
-    const Layer<Dual2> hs = layer(rows[L - 1]);
-    Dual2 e[2] = {hs.rho * m_sqrt((k + hs.kb) * m_fabs(k - hs.kb)), Dual2(1.)};
+    const Layer<GD> hs = layer(rows[L - 1]);
+    GD e[2] = {hs.rho * m_sqrt((k + hs.kb) * m_fabs(k - hs.kb)), GD(1.)};
     for (int j = L - 2; j >= top; --j) love_layer(layer(rows[j]), k, e);
     F = e[0];
   }

@@ -523,6 +523,10 @@ MDR_HD Dual2 rescale(Dual2 *e) {
 //    sqrt z, cosh sqrt -z) and S(z) = sum (-z)^n / (2n+1)! (sin sqrt z / sqrt
 //    z, sinh sqrt -z / sqrt -z) to n = 7 (truncation below 1e-18), so no 1/r
 //    appears; the closed forms lose about eps / p^2 of the derivative of w.
+// Narrow: the series only for |z| < 1e-4 (to n = 3, truncation below 1e-20),
+// and above it e^-p sinh p formed from m = expm1(-p) as -m (2 + m) / 2,
+// which keeps full relative precision for small p (1 - e^-2p loses eps / p
+// of it): fewer lanes of a wave straddle the two branches.
 // ---------------------------------------------------------------------------
 namespace mdr {
 
@@ -532,15 +536,17 @@ constexpr double kInvFact[16] = {1., 1., 0.5, 0.16666666666666666, 0.04166666666
                                  2.08767569878681e-09, 1.6059043836821613e-10, 1.1470745597729725e-11,
                                  7.647163731819816e-13};  // 1 / n!
 
-template <class D>
+template <class D, bool Narrow = false>
 MDR_HD Wave<D> wave_factor_held(D k, D kx, D h) {
+  constexpr double kZmax = Narrow ? 1e-4 : 0.25;
+  constexpr int kTop = Narrow ? 3 : 7;  // series to the z^kTop term
   Wave<D> o;
   const D r2s = (kx + k) * (kx - k);  // signed r^2: > 0 propagating
   const D z = r2s * h * h;
-  if (std::fabs(z.v) < 0.25) {
+  if (std::fabs(z.v) < kZmax) {
     const D u = -z;
-    D cs(kInvFact[14]), ss(kInvFact[15]);
-    for (int n = 6; n >= 0; --n) {
+    D cs(kInvFact[2 * kTop]), ss(kInvFact[2 * kTop + 1]);
+    for (int n = kTop - 1; n >= 0; --n) {
       cs = cs * u + D(kInvFact[2 * n]);
       ss = ss * u + D(kInvFact[2 * n + 1]);
     }
@@ -563,7 +569,18 @@ MDR_HD Wave<D> wave_factor_held(D k, D kx, D h) {
     D r, ir;
     root_pair(-r2s, &r, &ir);
     const D p = r * h;
-    const double e = std::exp(-p.v), f = e * e, cv = (1. + f) * 0.5, sv = (1. - f) * 0.5;
+    double e, cv, sv;
+    if (Narrow) {
+      const double m = std::expm1(-p.v);
+      e = 1. + m;
+      sv = -0.5 * m * (2. + m);
+      cv = 1. - sv;
+    } else {
+      e = std::exp(-p.v);
+      const double f = e * e;
+      cv = (1. + f) * 0.5;
+      sv = (1. - f) * 0.5;
+    }
     const D dp = p - D(p.v);  // the derivative part of p
     const D s = D(sv) + dp * cv;
     o.c = D(cv) + dp * sv;
