@@ -212,8 +212,10 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
   }
   const double lo = mi.lo, hi = mi.hi;
   // Far below the largest shear speed the FP32 compound matrix cancels
-  // (reference limit: max Vs / 16); such windows go straight to the FP64 grid.
-  if (fp32 && lo < mi.stable_low) {
+  // (reference limit: max Vs / 16). Where no root can lie down there the
+  // nodes below the limit are FP64-rechecked like any uncertain node;
+  // otherwise the window goes straight to the FP64 grid.
+  if (fp32 && lo < mi.stable_low && !mi.low_free) {
     if (lane == 0) {
       *ts = TraceScan{0, kTraceOverflow, 0, 0, 0, 0, 0u, 0u};
       a.overflow_list[atomicAdd(a.overflow_count, 1)] = tl;
@@ -295,7 +297,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
       const T k = T(omega / c);
       const T F = a.wave == 0 ? rayleigh_F_scale<T>(sh.lay, L, top, k, &sc) : love_F_scale<T>(sh.lay, L, top, k, &sc);
       f = float(F);
-      u = fp32 && (!(fabsf(f) >= tau) || c >= hi_guard || end);
+      u = fp32 && (!(fabsf(f) >= tau) || c >= hi_guard || end || c < mi.stable_low);
     }
     const float fl = __shfl(f, (lane + 63) & 63, 64), fr = __shfl(f, (lane + 1) & 63, 64);
     const float sl = __shfl(sc, (lane + 63) & 63, 64), sr = __shfl(sc, (lane + 1) & 63, 64);
@@ -309,7 +311,9 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
     if (classify) {
       if (u) kind = kUncertain;
       else if (has_right && !ur && signbit(f) != signbit(fr)) kind = kBracket;
-      if (has_left && has_right) {
+      // No probe next to FP32 values below the stable limit (meaningless
+      // there, and the band holds no root).
+      if (has_left && has_right && !(fp32 && cl < mi.stable_low)) {
         // A hidden root pair in a same-sign cell shows as a small local
         // minimum of |F| (trough) or as a dip of the log2 normalization scale
         // (a state vector that collapses between two very close roots). The
@@ -615,7 +619,9 @@ __global__ __launch_bounds__(kBlock) void unc_probe_kernel(PipeArgs a) {
       const int tl = g / kEventCap, e = g % kEventCap;
       const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
       const Event &x = ev[e];
-      if (!x.probe && x.c0 == x.c0 && x.c2 == x.c2) {
+      // The FP32 probe pass cannot work below the FP32 limit (a root-free band).
+      const double low = a.fp32 ? a.models[a.traces[tl + a.trace_begin].model].stable_low : 0.;
+      if (!x.probe && x.c0 == x.c0 && x.c2 == x.c2 && !(x.c0 < low)) {
         const NodeValues v = node_values(ev, e, a.scan[tl].nev);
         want = signbit(v.l) == signbit(v.c) && signbit(v.r) == signbit(v.c) && fabs(v.c) < fabs(v.l) &&
                fabs(v.c) < fabs(v.r) && fabs(v.c) < double(kTroughMax);

@@ -71,7 +71,7 @@ struct ModelInfo {
   double stable_low;    // FP32 nodes below this speed are rechecked in FP64
   double slowness;  // sum of h/vs over the finite layers (FP32 band sizing)
   int empty;       // 0 normal, 1 empty range, 2 no Love guide
-  int pad;
+  int low_free;    // 1 when no root can lie below stable_low (see make_model)
 };
 
 struct TraceInfo {
@@ -266,10 +266,11 @@ MDP_HD ModelInfo make_model_info(const double *h, const double *vp, const double
   mi.hi = hi;
   mi.vs_half = vs[L - 1];
   mi.cutoff_guard = vs[L - 1] * (1. - 1e-3);
-  double vmax = 0., s = 0.;
+  double vmax = 0., vmin = vs[L - 1], s = 0.;
   bool guide = false;
   for (int j = 0; j < L; ++j) {
     if (vs[j] > vmax) vmax = vs[j];
+    if (j >= mi.top && j < L - 1 && h[j] > 0. && vs[j] < vmin) vmin = vs[j];
     if (j < L - 1) {
       s += vs[j] > 0. ? h[j] / vs[j] : h[j] / vp[j];
       // A Love guide needs a finite solid layer slower than the half-space
@@ -278,8 +279,13 @@ MDP_HD ModelInfo make_model_info(const double *h, const double *vp, const double
     }
   }
   mi.stable_low = vmax / 16.;
+  // In a solid stack every Rayleigh or Love mode is faster than 0.68 of the
+  // slowest shear speed (the Rayleigh speed of a layer exceeds 0.68 vs for
+  // any vp/vs > sqrt(4/3)); with a margin of 2, no root lies below
+  // stable_low, so an FP32 scan only needs FP64 signs for the nodes there.
+  // A water top (Scholte modes) keeps the FP64 rescan of the whole trace.
+  mi.low_free = (!mi.top && mi.stable_low <= 0.5 * vmin) ? 1 : 0;
   mi.empty = !(hi > lo) ? 1 : ((wave == 1 && !guide) ? 2 : 0);
-  mi.pad = 0;
   mi.slowness = s;
   *slowness = s;
   return mi;

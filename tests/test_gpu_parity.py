@@ -99,6 +99,21 @@ def test_plain_workload_subset_automatic_window(md):
     assert_parity(got, ref, label="plain automatic")
 
 
+@pytest.mark.parametrize("wave", [0, 1], ids=["rayleigh", "love"])
+def test_precision_workload_subset_automatic_window(md, wave):
+    """The reference's automatic window reaches below max Vs / 16, where the
+    FP32 secular function is not trusted: those nodes take FP64 signs
+    (the band is root-free for these solid stacks) instead of the whole trace
+    going to the FP64 grid."""
+    from modisp_amd import workloads
+    wl = workloads.precision_batch(B=24, seed=103)
+    b = md.bounds(wl.thickness, wl.vp, wl.vs, wl.rho, wave)
+    assert np.all(b[:, 0] < wl.vs.max(axis=1) / 16)
+    got = gpu_solve(md, *(getattr(wl, k) for k in MODEL), wl.periods, b, 5, wave, "float32")
+    ref = oracle.solve(*(getattr(wl, k) for k in MODEL), wl.periods, b, 5, wave, automatic=True, tol=1e-12, group=2)
+    assert_parity(got, ref, label=f"precision automatic wave {wave}")
+
+
 def differing_traces(got, ref):
     with np.errstate(invalid="ignore"):
         rel = np.abs(got["phase_velocity"] - ref["phase_velocity"]) / ref["phase_velocity"]
