@@ -72,10 +72,30 @@ __device__ __forceinline__ Layer<T> make_layer(const PrepRow &r, double omega, d
   return l;
 }
 
+// Running product of normalization scales as mant 2^ex, mant in [1, 2) in
+// FP32, from the bits of each scale m (a normal positive double): its
+// exponent exactly, its leading 20 mantissa bits into the product. Integer
+// and FP32 operations only; one log2 at the end. The dropped low mantissa
+// bits perturb the product by < 1e-6 relative, smoothly enough for bracket
+// interpolation.
+struct ScaleProduct {
+  float mant = 1.f;
+  int ex = 0;
+  __device__ __forceinline__ void add(double m) {
+    const int hi = __double2hiint(m);
+    ex += (hi >> 20) - 1023;
+    mant *= __int_as_float(((hi & 0xfffff) << 3) | 0x3f800000);
+    const int carry = (__float_as_int(mant) >> 23) - 127;  // 0 or 1: mant in [1, 4)
+    ex += carry;
+    mant = __int_as_float(__float_as_int(mant) - (carry << 23));
+  }
+};
+
 // Secular F from prepared rows held in global memory (FP64 lane-per-item
-// work); optionally the summed log2 normalization scale.
+// work); optionally the summed log2 normalization scale (float or double sum).
+
 __device__ __forceinline__ double secular_rows(const PrepRow *rows, int L, int top, int wave, double omega,
-                                               double c, float *log_scale = nullptr) {
+                                               double c, float *log_scale = nullptr, ScaleProduct *product = nullptr) {
   const double k = omega / c, iw2 = 1. / (omega * omega);
   float ls = 0.f;
   double F;
@@ -84,9 +104,11 @@ __device__ __forceinline__ double secular_rows(const PrepRow *rows, int L, int t
     double e[5];
     double m = rayleigh_halfspace(make_layer<double>(rows[L - 1], omega, iw2, 0), k, k2, e);
     if (log_scale) ls += scale_log2(m);
+    if (product) product->add(m);
     for (int j = L - 2; j >= top; --j) {
       m = rayleigh_layer(make_layer<double>(rows[j], omega, iw2, 0), k, k2, e);
       if (log_scale) ls += scale_log2(m);
+      if (product) product->add(m);
     }
     F = top ? rayleigh_water_top(make_layer<double>(rows[0], omega, iw2, 0), k, e) : e[0];
   } else {
@@ -94,9 +116,11 @@ __device__ __forceinline__ double secular_rows(const PrepRow *rows, int L, int t
     double e[2] = {hs.rho * m_sqrt((k + hs.kb) * m_fabs(k - hs.kb)), 1.};
     double m = rescale<2>(e);
     if (log_scale) ls += scale_log2(m);
+    if (product) product->add(m);
     for (int j = L - 2; j >= top; --j) {
       m = love_layer(make_layer<double>(rows[j], omega, iw2, 1), k, e);
       if (log_scale) ls += scale_log2(m);
+      if (product) product->add(m);
     }
     F = e[0];
   }
@@ -833,8 +857,8 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(PipeArgs a) {
   const unsigned flags = ts.flags;
   if (ts.status == 0 && !(flags & kTraceOverflow)) {
     const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
-    auto emit = [&](double lo, double hi, double flo, double fhi) {
-      if (nb < K) out[nb] = Bracket{lo, hi, flo, fhi};
+    auto emit = [&](double lo, double hi, double flo, double fhi, float slo = NAN, float shi = NAN) {
+      if (nb < K) out[nb] = Bracket{lo, hi, flo, fhi, slo, shi};
       ++nb;
     };
     int last_split = -2;  // left node of the cell that holds the last split pair
@@ -854,12 +878,12 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(PipeArgs a) {
       // Left cell [c0, c1]: an uncertain node owns it when that neighbour is certain.
       if (split_left) emit_pair(x, x.node - 1, x.c0, x.c1, v.l, v.c);
       else if (x.kind == kUncertain && !v.lu && x.c0 == x.c0 && signbit(v.l) != signbit(v.c))
-        emit(x.c0, x.c1, v.l, v.c);
+        emit(x.c0, x.c1, v.l, v.c, x.sl, x.sc);
       if (nb >= K) break;
       // Right cell [c1, c2]: a certain sign change, or an uncertain node's.
       if (split && !split_left) emit_pair(x, x.node, x.c1, x.c2, v.c, v.r);
       else if (x.kind == kBracket || (x.kind == kUncertain && x.c2 == x.c2 && signbit(v.c) != signbit(v.r)))
-        emit(x.c1, x.c2, v.c, v.r);
+        emit(x.c1, x.c2, v.c, v.r, x.sc, x.sr);
     }
   }
   const int count = nb < K ? nb : K;
@@ -973,7 +997,7 @@ __global__ __launch_bounds__(kBlock) void love_phase_kernel(PipeArgs a) {
     if (m == 0) a.scan[tl].count = want;
     if (m >= want) continue;
     const double level = (floor(pa / pi) - double(m)) * pi;
-    Bracket br{NAN, NAN, 0., 0.};
+    Bracket br{NAN, NAN, 0., 0., NAN, NAN};
     double fa = 0., fb = 0.;
     bool have_f = false;
     for (int it = 0; it < 200; ++it) {
@@ -988,7 +1012,7 @@ __global__ __launch_bounds__(kBlock) void love_phase_kernel(PipeArgs a) {
       if (!(p == p)) break;
       if (p > level) { lo = mid; pa = p; } else { hi = mid; pb = p; }
     }
-    if (have_f) br = Bracket{lo, hi, fa, fb};
+    if (have_f) br = Bracket{lo, hi, fa, fb, NAN, NAN};
     a.brackets[static_cast<size_t>(tl) * a.kmax + m] = br;
   }
 }
@@ -1147,6 +1171,21 @@ __device__ double refine_serial(int method, const PrepRow *rows, int L, int top,
   return root;
 }
 
+// f 2^d with d clamped to +-600 bits, never rounding a nonzero f to zero (a
+// zero ends the refinement as an exact root).
+__device__ __forceinline__ double unscale(double f, double d) {
+  const double g = f * exp2(fmin(600., fmax(-600., d)));
+  return g == 0. && f != 0. ? copysign(4.9406564584124654e-324, f) : g;
+}
+// f mant 2^(ex - ref) for a scale product: the integer exponent by ldexp,
+// the fraction of ref by one FP32 exp2.
+__device__ __forceinline__ double unscale(double f, const ScaleProduct &p, float ref) {
+  const float ri = floorf(ref);
+  const int e = min(600, max(-600, p.ex - int(ri)));
+  const double g = ldexp(f * double(p.mant * exp2f(ri - ref)), e);
+  return g == 0. && f != 0. ? copysign(4.9406564584124654e-324, f) : g;
+}
+
 // One instantiation per method (the method is the same for the whole launch).
 template <int Method>
 __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
@@ -1154,12 +1193,17 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
   const int total = ntl * a.kmax;
   const int stride = gridDim.x * kBlock;
   int q = blockIdx.x * kBlock + threadIdx.x - stride;  // advanced before each fetch
-  RefineState s{};
+  // The lane's state lives in LDS, so that the evaluation of F has the
+  // registers to itself.
+  __shared__ RefineState state[kBlock];
+  RefineState &s = state[threadIdx.x];
+  s = RefineState{};
   bool active = false;
   unsigned evals = 0;
   const PrepRow *rows = nullptr;
   int L = 0, top = 0;
   double omega = 0.;
+  float ref = NAN;  // reference log2 scale of the unnormalized F; NaN: refine F itself
   bool exhausted = false;
   // Each pass: lanes fetch brackets and take Brent steps until they need one
   // evaluation of F (or run out of work); then the wave evaluates together.
@@ -1173,13 +1217,20 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
         const int count = a.scan[tl].count;
         const TraceInfo ti = a.traces[tl + a.trace_begin];
         const ModelInfo mi = a.models[ti.model];
-        const Bracket br = m < count && m < mi.K ? a.brackets[q] : Bracket{NAN, NAN, 0., 0.};
+        const Bracket br = m < count && m < mi.K ? a.brackets[q] : Bracket{NAN, NAN, 0., 0., NAN, NAN};
         if (!(br.lo == br.lo)) { a.roots[q] = NAN; continue; }
         if (br.flo == 0.) { a.roots[q] = br.lo; continue; }
         if (br.fhi == 0.) { a.roots[q] = br.hi; continue; }
         // Guard roots beyond the requested modes only feed the close-root test.
         const double tol = m < mi.modes ? a.tol : fmax(a.tol, kGuardTol);
-        refine_init(s, Method, br.lo, br.hi, br.flo, br.fhi, tol);
+        // With both end scales known, refine G = F 2^(ls - ref): the same
+        // zeros (a positive factor), and smooth where the normalized F of a
+        // deep evanescent stack is a +-1 step that defeats interpolation.
+        const bool scaled = fabsf(br.slo) < INFINITY && fabsf(br.shi) < INFINITY;
+        ref = scaled ? fmaxf(br.slo, br.shi) : NAN;
+        const double glo = scaled ? unscale(br.flo, double(br.slo) - double(ref)) : br.flo;
+        const double ghi = scaled ? unscale(br.fhi, double(br.shi) - double(ref)) : br.fhi;
+        refine_init(s, Method, br.lo, br.hi, glo, ghi, tol);
         rows = a.rows + mi.row0;
         L = mi.L;
         top = mi.top;
@@ -1194,7 +1245,12 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
     if (!__ballot(active)) break;
     if (!active) continue;
     ++evals;
-    const double f = secular_rows(rows, L, top, a.wave, omega, refine_point(s, Method));
+    const double x = refine_point(s, Method);
+    __asm__ volatile("" ::: "memory");  // keep no copy of the state in registers across the evaluation
+    ScaleProduct sp;
+    double f = secular_rows(rows, L, top, a.wave, omega, x, nullptr, &sp);
+    if (ref == ref) f = unscale(f, sp, ref);
+    __asm__ volatile("" ::: "memory");  // reload the state from LDS after the evaluation
     if (!(f == f) || fabs(f) == INFINITY) {
       a.roots[q] = NAN;
       active = false;

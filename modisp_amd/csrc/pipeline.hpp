@@ -114,8 +114,14 @@ struct TraceScan {
   unsigned umask, pmask;  // events (bit e) that are FP32-uncertain / probed for a hidden pair
 };
 
+// A root bracket with F at its ends and the summed log2 normalization scales
+// there (NaN where unknown). Refinement works on F 2^(ls - max(slo, shi)):
+// the unnormalized secular function up to a constant, whose zero is F's but
+// which stays smooth where a deep evanescent stack turns the normalized F
+// into a +-1 step.
 struct Bracket {
   double lo, hi, flo, fhi;
+  float slo, shi;
 };
 
 struct PipeArgs {

@@ -170,6 +170,34 @@ def test_sweep_workload_subset(md):
         assert_parity(got, rest, label=f"sweep wave {wave}")
 
 
+def test_deep_stack_refinement(md):
+    """Deep random-walk stacks at short periods, where the max-normalized F of
+    the evanescent stack is a +-1 step across most brackets: refinement on the
+    unnormalized F (end scales from the scan) finds the oracle's roots, and in
+    few evaluations (a plain Brent on the step takes about 17 per bracket)."""
+    from modisp_amd import workloads
+    wl = workloads.sweep_batch(B=32, seed=106)
+    per = wl.periods[:40:4]
+    for wave in (0, 1):
+        b = md.bounds(wl.thickness, wl.vp, wl.vs, wl.rho, wave)
+        win = np.stack([np.maximum(b[:, 0], 0.8 * wl.vs.min(axis=1)), b[:, 1]], axis=1)
+        h, vp, vs, rho = (np.ascontiguousarray(getattr(wl, k)) for k in MODEL)
+        with md.QeSolver(wl.B, h.shape[1], per.size, 4, group_velocity=True, scan="float32") as s:
+            vel, flags, st, lev, cnt, ex = s.run(h, vp, vs, rho, per, win, 1e-12, 1., wave, False)
+            stats = s.last_stats()
+        got = dict(phase_velocity=vel, trace_flags=flags, search_level=lev, root_count=cnt,
+                   group_velocity=ex["group_velocity"], group_velocity_flags=ex["group_velocity_flags"])
+        per_bracket = stats["refine_evaluations"] / max(1, cnt.clip(0).sum())
+        assert per_bracket < 8., f"wave {wave}: {per_bracket:.1f} evaluations per bracket"
+        ref = oracle.solve(h, vp, vs, rho, per, win, 4, wave, tol=1e-12, group=2)
+        differ = differing_traces(got, ref)
+        assert differ.sum() <= max(1, 0.01 * differ.size), f"deep wave {wave}: {differ.sum()} traces differ"
+        for i, f in np.argwhere(differ):
+            assert_extra_pairs_are_sign_changes(wl, per, win, got, i, f, 4, wave)
+        rest = dict(ref, root_count=np.where(differ, -1, ref["root_count"]))
+        assert_parity(got, rest, label=f"deep wave {wave}")
+
+
 def test_ragged_uneven_batch(md):
     """Ragged models (3..40 finite layers, low-velocity zones, per-model
     period and mode counts) through the ragged entry point."""
