@@ -1054,6 +1054,16 @@ __device__ __forceinline__ void refine_init(RefineState &s, int method, double l
   s.fresh = false;
 }
 
+// The root from a converged bracket [a, b] with end values of opposite sign:
+// the secant point, which inside a bracket this narrow (the refinement stop
+// width) is the root to rounding, so that quantities steep in c (the group
+// velocity near a body-wave speed) are evaluated at the root itself; the
+// midpoint where the secant is unusable.
+__device__ __forceinline__ double bracket_root(double a, double b, double fa, double fb) {
+  const double x = a - fa * ((b - a) / (fb - fa));
+  return x >= fmin(a, b) && x <= fmax(a, b) ? x : .5 * (a + b);
+}
+
 __device__ __forceinline__ void refine_take(RefineState &s, double f) {
   s.fx = f;
   s.fresh = true;
@@ -1065,7 +1075,7 @@ __device__ __forceinline__ int brent_step(RefineState &s, double &root) {
   if (fabs(s.fc) < fabs(s.fb)) { s.a = s.b; s.fa = s.fb; s.b = s.c; s.fb = s.fc; s.c = s.a; s.fc = s.fa; }
   if (s.fb == 0.) { root = s.b; return 1; }
   const double mid = .5 * (s.c - s.b), step = .5 * s.stop;
-  if (fabs(s.c - s.b) <= s.stop) { root = .5 * (s.b + s.c); return 1; }
+  if (fabs(s.c - s.b) <= s.stop) { root = bracket_root(s.b, s.c, s.fb, s.fc); return 1; }
   if (s.iter >= 160) { root = NAN; return 1; }
   if (fabs(s.e) >= step && fabs(s.fa) > fabs(s.fb)) {
     const double sr = s.fb / s.fa;
@@ -1100,7 +1110,7 @@ __device__ __forceinline__ int chandrupatla_step(RefineState &s, double &root) {
     if ((f < 0.) == (s.fa < 0.)) { s.c = s.a; s.fc = s.fa; } else { s.b = s.a; s.fb = s.fa; }
     s.a = s.x; s.fa = f;
     const double dx = fabs(s.b - s.a);
-    if (dx <= s.stop) { root = .5 * (s.a + s.b); return 1; }
+    if (dx <= s.stop) { root = bracket_root(s.a, s.b, s.fa, s.fb); return 1; }
     // Inverse quadratic step when the three points admit it (Chandrupatla's test).
     const double xi = (s.a - s.b) / (s.c - s.b), phi = (s.fa - s.fb) / (s.fc - s.fb);
     double t = .5;
@@ -1112,7 +1122,7 @@ __device__ __forceinline__ int chandrupatla_step(RefineState &s, double &root) {
     const double limit = fmin(.5, .5 * s.stop / dx);
     s.d = fmax(limit, fmin(1. - limit, t));
   }
-  if (fabs(s.b - s.a) <= s.stop) { root = .5 * (s.a + s.b); return 1; }
+  if (fabs(s.b - s.a) <= s.stop) { root = bracket_root(s.a, s.b, s.fa, s.fb); return 1; }
   if (s.iter >= 160) { root = NAN; return 1; }
   double x = s.a + s.d * (s.b - s.a);
   if (!(x > fmin(s.a, s.b) && x < fmax(s.a, s.b))) x = .5 * (s.a + s.b);
@@ -1129,7 +1139,7 @@ __device__ __forceinline__ int legacy_step(RefineState &s, double &root) {
     if ((f < 0.) == (s.fa < 0.)) { s.a = s.x; s.fa = f; } else { s.b = s.x; s.fb = f; }
   }
   const double w = s.b - s.a;
-  if (!(s.iter < 100 && w > s.stop)) { root = w <= s.stop ? .5 * (s.a + s.b) : NAN; return 1; }
+  if (!(s.iter < 100 && w > s.stop)) { root = w <= s.stop ? bracket_root(s.a, s.b, s.fa, s.fb) : NAN; return 1; }
   double x = s.a + w * (fabs(s.fa) / (fabs(s.fa) + fabs(s.fb)));
   if (s.iter % 3 == 2 || !(fabs(x) < INFINITY) || x <= s.a + .1 * w || x >= s.b - .1 * w) x = .5 * (s.a + s.b);
   s.x = x;

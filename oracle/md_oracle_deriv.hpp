@@ -321,9 +321,19 @@ inline Dual secular_direction(const double *h, const double *vp, const double *v
 }
 
 // Analytic implicit group velocity U = d(omega)/dk = -F_k / F_omega at a root.
+// At a root c from bracket refinement (accurate to the refinement tolerance,
+// ~1e-13 relative) the group velocity is evaluated after one Newton step in k
+// on F: near a body-wave speed U is steep enough in c that the tolerance alone
+// would move it by 1e-8; the step lands on the root to rounding. A step larger
+// than 1e-10 relative (c not at a root) is not taken.
 inline double group_velocity_analytic(const double *h, const double *vp, const double *vs, const double *rho, int L,
                                       int wave, double omega, double c) {
-  const Dual fk = secular_direction(h, vp, vs, rho, L, wave, omega, c, 0);
+  Dual fk = secular_direction(h, vp, vs, rho, L, wave, omega, c, 0);
+  const double k = omega / c, kn = k - fk.v / fk.d, cn = omega / kn;
+  if (std::isfinite(cn) && std::fabs(cn - c) <= 1e-10 * c && cn != c) {
+    c = cn;
+    fk = secular_direction(h, vp, vs, rho, L, wave, omega, c, 0);
+  }
   const Dual fw = secular_direction(h, vp, vs, rho, L, wave, omega, c, 1);
   return -fk.d / fw.d;
 }
