@@ -594,6 +594,7 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(PipeArgs a) {
 // ---------------------------------------------------------------------------
 // FP64 recheck of uncertain nodes (dense lanes over the device queue).
 // ---------------------------------------------------------------------------
+template <int Wave>  // per wave type: the Love instantiation needs far fewer registers
 __global__ __launch_bounds__(kBlock) void recheck_kernel(PipeArgs a) {
   const int n = min(*a.unc_count, a.queue_cap);
   for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
@@ -602,7 +603,7 @@ __global__ __launch_bounds__(kBlock) void recheck_kernel(PipeArgs a) {
     Event &x = a.events[g];
     const TraceInfo ti = a.traces[tl + a.trace_begin];
     const ModelInfo mi = a.models[ti.model];
-    x.v = secular_rows(a.rows + mi.row0, mi.L, mi.top, a.wave, ti.omega, x.c1);
+    x.v = secular_rows(a.rows + mi.row0, mi.L, mi.top, Wave, ti.omega, x.c1);
   }
 }
 
@@ -732,7 +733,7 @@ constexpr int kProbeNeeds64 = 0x100;
 // probe of the FP64 path, or the handed-over probes of the FP32 path, in FP64;
 // a resumed search restarts the golden section on the handed-over bracket (one
 // extra evaluation: the retained interior point sits at a golden point of it).
-template <class T>
+template <class T, int Wave>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void trough_kernel(PipeArgs a) {
   constexpr bool fp32 = sizeof(T) == 4;
   // The FP64 pass of the FP32 path takes the compact list of handed-over probes.
@@ -805,7 +806,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       const double cs = snap32(omega, s.pending, s.lo, s.hi);
       bool handed = !(cs < guard && s.hi - s.lo > 2e-6 * s.hi && cs > s.lo && cs < s.hi);
       if (!handed) {
-        const float f = secular_global<float>(rows, L, top, a.wave, omega, iw2, float(omega / cs), &ls);
+        const float f = secular_global<float>(rows, L, top, Wave, omega, iw2, float(omega / cs), &ls);
         handed = !(fabsf(f) >= tau);
         if (!handed) {
           const double sf = s.sign * double(f);
@@ -834,7 +835,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         }
       }
     } else {
-      const double f = s.sign * secular_rows(rows, L, top, a.wave, omega, s.pending, &ls);
+      const double f = s.sign * secular_rows(rows, L, top, Wave, omega, s.pending, &ls);
       golden_take(s, s.pending, f, s.dip ? ls : float(f));
     }
   }
@@ -1196,8 +1197,9 @@ __device__ __forceinline__ double unscale(double f, const ScaleProduct &p, float
   return g == 0. && f != 0. ? copysign(4.9406564584124654e-324, f) : g;
 }
 
-// One instantiation per method (the method is the same for the whole launch).
-template <int Method>
+// One instantiation per method and wave type (both fixed for a launch): the
+// Love instantiation is not held to the Rayleigh evaluation's registers.
+template <int Method, int Wave>
 __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
   const int ntl = a.trace_end - a.trace_begin;
   const int total = ntl * a.kmax;
@@ -1258,7 +1260,7 @@ __global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
     const double x = refine_point(s, Method);
     __asm__ volatile("" ::: "memory");  // keep no copy of the state in registers across the evaluation
     ScaleProduct sp;
-    double f = secular_rows(rows, L, top, a.wave, omega, x, nullptr, &sp);
+    double f = secular_rows(rows, L, top, Wave, omega, x, nullptr, &sp);
     if (ref == ref) f = unscale(f, sp, ref);
     __asm__ volatile("" ::: "memory");  // reload the state from LDS after the evaluation
     if (!(f == f) || fabs(f) == INFINITY) {
@@ -1406,7 +1408,10 @@ __device__ __forceinline__ double group_at(const PrepRow *rows, int L, int top, 
   return -F.a / F.b;
 }
 
-__global__ __launch_bounds__(kBlock) void group_kernel(PipeArgs a) {
+// Rayleigh (0) and Love (1) get their own register allocations; two waves per
+// SIMD bind only the Rayleigh one (a few spills, still faster than one wave).
+template <int Wave>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void group_kernel(PipeArgs a) {
   const int ntl = a.trace_end - a.trace_begin;
   // Mode-major: neighbouring lanes take one mode of neighbouring traces (the
   // next periods of one model), whose roots sit in the same layer regimes, so
@@ -1430,7 +1435,7 @@ __global__ __launch_bounds__(kBlock) void group_kernel(PipeArgs a) {
                          (m + 1 < ts.count && roots[m + 1] - c <= a.close_rtol * roots[m + 1]);
       if (close) gf = kGroupClose;
       else {
-        u = group_at(a.rows + mi.row0, mi.L, mi.top, a.wave, ti.omega, c);
+        u = group_at(a.rows + mi.row0, mi.L, mi.top, Wave, ti.omega, c);
         if (!(u == u) || fabs(u) == INFINITY) { u = NAN; gf = kGroupNonfinite; }
       }
     }
@@ -1828,6 +1833,13 @@ hipError_t launch_gather(const TraceInfo *all, const int *list, int n, TraceInfo
   return hipGetLastError();
 }
 
+template <int Wave>
+static void launch_refine(const PipeArgs &pa, int pb, hipStream_t s) {
+  if (pa.refinement == 2) hipLaunchKernelGGL((refine_kernel<2, Wave>), dim3(pb), dim3(kBlock), 0, s, pa);
+  else if (pa.refinement == 0) hipLaunchKernelGGL((refine_kernel<0, Wave>), dim3(pb), dim3(kBlock), 0, s, pa);
+  else hipLaunchKernelGGL((refine_kernel<1, Wave>), dim3(pb), dim3(kBlock), 0, s, pa);
+}
+
 hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bool group, int persistent_blocks,
                            hipStream_t s, hipEvent_t *marks) {
   const int n = sa.trace_end - sa.trace_begin;
@@ -1853,10 +1865,16 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
   }
   mark(2);
   hipLaunchKernelGGL(compact_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
-  hipLaunchKernelGGL(recheck_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+  if (pa.wave == 1) hipLaunchKernelGGL(recheck_kernel<1>, dim3(pb), dim3(kBlock), 0, s, pa);
+  else hipLaunchKernelGGL(recheck_kernel<0>, dim3(pb), dim3(kBlock), 0, s, pa);
   if (fp32) hipLaunchKernelGGL(unc_probe_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
-  if (fp32) hipLaunchKernelGGL(trough_kernel<float>, dim3(pb), dim3(kBlock), 0, s, pa);
-  hipLaunchKernelGGL(trough_kernel<double>, dim3(pb), dim3(kBlock), 0, s, pa);
+  if (pa.wave == 1) {
+    if (fp32) hipLaunchKernelGGL((trough_kernel<float, 1>), dim3(pb), dim3(kBlock), 0, s, pa);
+    hipLaunchKernelGGL((trough_kernel<double, 1>), dim3(pb), dim3(kBlock), 0, s, pa);
+  } else {
+    if (fp32) hipLaunchKernelGGL((trough_kernel<float, 0>), dim3(pb), dim3(kBlock), 0, s, pa);
+    hipLaunchKernelGGL((trough_kernel<double, 0>), dim3(pb), dim3(kBlock), 0, s, pa);
+  }
   mark(3);
   hipLaunchKernelGGL(assemble_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   if (pa.wave == 1) {
@@ -1864,12 +1882,12 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
     hipLaunchKernelGGL(love_phase_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
   }
   mark(4);
-  if (pa.refinement == 2) hipLaunchKernelGGL(refine_kernel<2>, dim3(pb), dim3(kBlock), 0, s, pa);
-  else if (pa.refinement == 0) hipLaunchKernelGGL(refine_kernel<0>, dim3(pb), dim3(kBlock), 0, s, pa);
-  else hipLaunchKernelGGL(refine_kernel<1>, dim3(pb), dim3(kBlock), 0, s, pa);
+  if (pa.wave == 1) launch_refine<1>(pa, pb, s);
+  else launch_refine<0>(pa, pb, s);
   hipLaunchKernelGGL(finalize_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   mark(5);
-  if (group) hipLaunchKernelGGL(group_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+  if (group && pa.wave == 1) hipLaunchKernelGGL(group_kernel<1>, dim3(pb), dim3(kBlock), 0, s, pa);
+  else if (group) hipLaunchKernelGGL(group_kernel<0>, dim3(pb), dim3(kBlock), 0, s, pa);
   mark(6);
   if (pa.stats) {
     hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(64), 0, s, pa.counters, pa.stats, n);
