@@ -170,6 +170,32 @@ def test_sweep_workload_subset(md):
         assert_parity(got, rest, label=f"sweep wave {wave}")
 
 
+def test_roots_sit_on_sign_changes(md):
+    """Refinement ends on the secant point of its final bracket, which is the
+    root to rounding: F (the oracle's evaluator) changes sign within 1e-14
+    relative either side of the device root (589 of 590 roots here). The
+    bracket midpoint, anywhere within half the 1.25e-13 stop width, passes
+    this for 18 of the 590."""
+    from modisp_amd import workloads
+    wl = workloads.precision_batch(B=8, seed=107)
+    per = wl.periods[::10]
+    win = workloads.scan_window(wl.vs)
+    hits = total = 0
+    for wave in (0, 1):
+        got = gpu_solve(md, *(getattr(wl, k) for k in MODEL), per, win, 5, wave, "float32", group=False)
+        for i in range(wl.thickness.shape[0]):
+            model = [np.ascontiguousarray(getattr(wl, k)[i]) for k in MODEL]
+            for f in range(per.size):
+                for r in got["phase_velocity"][i, :, f]:
+                    if not np.isfinite(r):
+                        continue
+                    fl, fr = oracle.secular(*model, per[f], np.array([r * (1 - 1e-14), r * (1 + 1e-14)]), wave=wave)
+                    total += 1
+                    hits += np.signbit(fl) != np.signbit(fr)
+    assert total > 300
+    assert hits >= 0.99 * total, f"{hits} of {total} roots within 1e-14 of a sign change"
+
+
 def test_deep_stack_refinement(md):
     """Deep random-walk stacks at short periods, where the max-normalized F of
     the evanescent stack is a +-1 step across most brackets: refinement on the
