@@ -213,7 +213,7 @@ def gradient_leg(torch, modisp_amd, workloads, steps, warmup):
     dev = torch.device("cuda", 0)
     B, L = wl.thickness.shape
     window = workloads.scan_window(wl.vs)
-    solver = modisp_amd.QeSolver(B, L, wl.periods.size, 3)
+    solver = modisp_amd.QeSolver(B, L, wl.periods.size, 3, scan="float32")
     per = torch.tensor(wl.periods, device=dev)
     win = torch.tensor(window, device=dev)
     with torch.no_grad():
@@ -272,7 +272,7 @@ def uneven_leg(modisp_amd, workloads, models):
     bounds = np.array([[0.8 * wl.vs[i, :lay[i]].min(), np.nextafter(wl.vs[i, lay[i] - 1], 0.)] for i in range(wl.B)])
     out = {}
     for cont in (False, True):
-        with modisp_amd.QeSolver(wl.B, int(lay.max()), 1, 1, continuation=cont) as s:
+        with modisp_amd.QeSolver(wl.B, int(lay.max()), 1, 1, continuation=cont, scan="float32") as s:
             s.run_ragged(*cols, row_off, per, per_off, wl.nmodes, bounds, 1e-12, 0)      # warm-up
             t0 = time.perf_counter()
             r = s.run_ragged(*cols, row_off, per, per_off, wl.nmodes, bounds, 1e-12, 0)
@@ -322,7 +322,7 @@ def sweep_leg(torch, modisp_amd, workloads, steps):
     per = torch.tensor(wl.periods, device=dev)
     win = torch.tensor(window, device=dev)
     L = wl.thickness.shape[1]
-    solvers = [modisp_amd.QeSolver(B, L, wl.periods.size, 4) for _ in (0, 1)]
+    solvers = [modisp_amd.QeSolver(B, L, wl.periods.size, 4, scan="float32") for _ in (0, 1)]
     with torch.no_grad():
         tm = [torch.tensor(x, device=dev) for x in (wl.thickness, wl.vp, target_vs, wl.rho)]
         tgt = [phase_velocity(*tm, per, win, nmodes=4, wave=w, solver=solvers[w]) for w in (0, 1)]

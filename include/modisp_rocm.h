@@ -33,11 +33,16 @@ typedef struct md_rocm_solver md_rocm_solver;
  * group_velocity, group_rtol, continuation, anchor_stride; plus this
  * backend's grid size and FP32 uncertainty band. */
 typedef struct md_rocm_config {
-  int accuracy;            /* 0 high (exhaustive grid + trough probes); others accepted */
+  int accuracy;            /* reference preset code: 0 high, 1 balanced, 2 fast, 4 auto (3, the legacy
+                              uniform path, is grid = 0 here). Every preset runs this backend's exhaustive
+                              grid scan with hidden-pair probes; the preset sets the reported contract:
+                              search_level (with `shape`) and TRACE_APPROXIMATE_SEARCH (2048) on every
+                              searched trace of balanced, fast and auto, as the reference reports them */
   double close_root_rtol;  /* close-root threshold, default 0.005 */
   int single_precision;    /* 1: FP32 grid scan, FP64 recheck/refine; 0: FP64 grid scan */
   int group_velocity;      /* 1: analytic FP64 group velocity */
-  double group_rtol;       /* accepted for API parity; the analytic derivative is exact */
+  double group_rtol;       /* group velocities whose rounding-error bound exceeds 0.25 group_rtol
+                              max(c, |U|) are flagged DERIVATIVE_UNSTABLE (group_velocity.hpp:87-92) */
   int continuation;        /* 1: anchor continuation across periods on the host-array entry points
                               (md_rocm_run, md_rocm_run_ragged): anchors every <= anchor_stride periods
                               (log step <= 0.25) by the full pipeline, the rest predicted, refined and
@@ -57,6 +62,22 @@ typedef struct md_rocm_config {
   int consumer_blocks;     /* persistent consumer blocks per compute unit, 1..64 (0 = 8) */
   int refinement;          /* bracket refinement (reference QeSolver refinement, refinement.hpp:11-65):
                               0 legacy regula falsi, 1 Brent-Dekker (default), 2 Chandrupatla */
+  int shape;               /* reference `shape` (search_strategy shape_adaptive, the default 1, or
+                              qe_adaptive, 0): selects the search_level codes (8/9/10 against 3/6/7,
+                              docs/API.md:203-209) and, as the reference's searches do, whether close
+                              roots of a Rayleigh trace raise TRACE_CLOSE_ROOT_RISK (4096): the shape and
+                              observation searches test them (cpp/cpu/solver.cpp:196,145), the high
+                              adaptive ladder does not */
+  int love_phase;          /* 1 (default): Love roots verified by the exact phase count and re-bracketed
+                              by phase bisection where the grid disagrees (love_phase.hpp:76-91), level
+                              11; 0: the grid scan alone (reference love_strategy="scan") */
+  int secular;             /* 1 (fast, the contracted Dunkin form, dunkin.hpp:459-566): the only form
+                              built here; 0 (the released 25-entry form) is refused */
+  double sampling_rtol;    /* validated as the reference (0 < x <= 1, or -1 for auto sampling) and kept
+                              for reports: it loosens the reference's shape sampling, which this
+                              backend's fixed-density scan does not use */
+  double period_threshold; /* validated (finite, >= 0) and kept for reports; the reference also keeps it
+                              as metadata only (cpp/common/accuracy.hpp:47-53) */
 } md_rocm_config;
 
 /* Defaults for every field. */
@@ -71,7 +92,10 @@ md_rocm_solver *md_rocm_create(int capacity, int L, int nf, int nmodes, const md
 void md_rocm_free(md_rocm_solver *s);
 
 /* Solve B <= capacity models from HOST arrays. Replaces QeSolver.run
- * (cpp/cuda/qe.cu:1060-1118):
+ * (cpp/cuda/qe.cu:1060-1118). f_accept < 1 rejects reported roots whose
+ * max-normalised |F| exceeds it: NaN velocity, TRACE_F_REJECTED (128), group
+ * flag MISSING_ROOT (reference cpp/cpu/solver.cpp:333); a continued target
+ * that fails it falls back to the full search (continuation.hpp:201-204).
  *   thickness, vp, vs, rho : [B, L] row-major float64
  *   periods                : [nf] shared, or [B, nf] when row_periods != 0
  *   bounds                 : [B, 2] search window (lo, hi)

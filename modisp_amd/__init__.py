@@ -205,22 +205,41 @@ class QeSolver:
     """Persistent native engine; the constructor and run() mirror the reference
     QeSolver (cpp/cuda/qe.cu:1166 signature, run at qe.cu:1060-1118).
 
-    ``single_precision`` keeps the reference meaning of the returned dtype
-    (float32 velocities). Extra keyword-only arguments select this backend's
-    method: ``scan`` ("float32": FP32 grid scan with FP64 recheck and FP64
-    refinement, the default; "float64": the plain FP64 grid path), ``grid``
+    Reference arguments and what they do here:
+
+    * ``single_precision``: the evaluation precision of the grid scan and the
+      dtype of returned velocities, as in the reference (True: FP32 scan with
+      FP64 recheck, refinement and group velocity, float32 velocities;
+      False: the plain FP64 path, float64 velocities). ``scan`` overrides the
+      scan precision alone (``scan="float32"`` with float64 velocities is the
+      FP32-scan + FP64 product path).
+    * ``accuracy`` (0 high, 1 balanced, 2 fast, 4 auto) and ``shape``: every
+      preset runs the same exhaustive grid scan; together they set the
+      reported search_level codes and TRACE_APPROXIMATE_SEARCH as the
+      reference's searches report them (see include/modisp_rocm.h).
+    * ``refinement`` (0 legacy, 1 Brent, 2 Chandrupatla) and ``love_phase``
+      (1 exact phase count, 0 scan only) select those methods.
+    * ``secular``: 1 (the contracted form, the only one built); 0 raises.
+    * ``group_rtol``: analytic group velocities whose rounding bound exceeds
+      0.25 group_rtol max(c, |U|) are flagged DERIVATIVE_UNSTABLE.
+    * ``sampling_rtol`` and ``period_threshold``: validated and reported; the
+      reference's shape sampling they tune is not this backend's scan.
+
+    Extra keyword-only arguments select this backend's grid: ``grid``
     ("adaptive": nodes uniform in the WKB mode count with ``rho`` nodes per
     mode over a 64-point floor, the default; "uniform": ``nc`` equally spaced
     nodes) and the FP32 uncertainty band (``tau_floor``, ``tau_phase``).
     """
 
-    def __init__(self, B, L, nf, nmodes, accuracy=0, period_threshold=0., close_root_rtol=.005, shape=False,
-                 sampling_rtol=.001, secular=0, refinement=1, love_phase=1, single_precision=False,
-                 group_velocity=False, group_rtol=.001, continuation=False, anchor_stride=4, *, scan="float32",
+    def __init__(self, B, L, nf, nmodes, accuracy=0, period_threshold=0., close_root_rtol=.005, shape=True,
+                 sampling_rtol=.001, secular=1, refinement=1, love_phase=1, single_precision=False,
+                 group_velocity=False, group_rtol=.001, continuation=False, anchor_stride=4, *, scan=None,
                  grid="adaptive", rho=8., nc=512, tau_floor=None, tau_phase=None, chunk_traces=0, graph=False,
                  scan_waves=0, consumer_blocks=0):
         if grid not in ("adaptive", "uniform"):
             raise ValueError("grid must be adaptive or uniform")
+        if scan is None:
+            scan = "float32" if single_precision else "float64"
         if scan not in ("float32", "float64"):
             raise ValueError("scan must be float32 or float64")
         cfg = _native.Config()
@@ -245,6 +264,11 @@ class QeSolver:
         cfg.scan_waves = int(scan_waves)
         cfg.consumer_blocks = int(consumer_blocks)
         cfg.refinement = int(refinement)
+        cfg.shape = int(bool(shape))
+        cfg.love_phase = int(love_phase)
+        cfg.secular = int(secular)
+        cfg.sampling_rtol = float(sampling_rtol)
+        cfg.period_threshold = float(period_threshold)
         self.capacity, self.L, self.nf, self.nmodes = int(B), int(L), int(nf), int(nmodes)
         self.single_precision = bool(single_precision)
         self.scan = scan

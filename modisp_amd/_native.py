@@ -25,7 +25,8 @@ class Config(ctypes.Structure):
                 ("nc", ctypes.c_int), ("tau_floor", ctypes.c_double), ("tau_phase", ctypes.c_double),
                 ("chunk_traces", ctypes.c_int), ("grid", ctypes.c_int), ("rho", ctypes.c_double),
                 ("graph", ctypes.c_int), ("scan_waves", ctypes.c_int), ("consumer_blocks", ctypes.c_int),
-                ("refinement", ctypes.c_int)]
+                ("refinement", ctypes.c_int), ("shape", ctypes.c_int), ("love_phase", ctypes.c_int),
+                ("secular", ctypes.c_int), ("sampling_rtol", ctypes.c_double), ("period_threshold", ctypes.c_double)]
 
 
 class KernelReport(ctypes.Structure):

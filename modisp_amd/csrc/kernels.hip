@@ -896,10 +896,13 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(PipeArgs a) {
 // cpp/common/love_phase.hpp:9-36). Roots are the c where the phase crosses
 // n*pi; the phase decreases with c.
 // ---------------------------------------------------------------------------
-__device__ double love_phase_dev(const PrepRow *rows, int L, int top, double omega, double c) {
+// vs_half: the half-space shear speed, so that the half-space wavenumber is
+// omega / vs rounded once, as the reference forms it: whether k > khalf holds
+// at the top of the window (nextafter(vs_half, 0)) decides if the phase exists.
+__device__ double love_phase_dev(const PrepRow *rows, int L, int top, double omega, double c, double vs_half) {
   constexpr double pi = 3.141592653589793;
   const int last = L - 1;
-  const double k = omega / c, khalf = omega * rows[last].ivs;
+  const double k = omega / c, khalf = omega / vs_half;
   if (!(k > khalf)) return NAN;
   double scale = 0.5 * rows[last].rho * rows[last].g2 * sqrt((k + khalf) * (k - khalf));
   double theta = .25 * pi;
@@ -956,8 +959,8 @@ __global__ __launch_bounds__(kBlock) void love_count_kernel(PipeArgs a) {
   const TraceInfo ti = a.traces[t];
   const ModelInfo mi = a.models[ti.model];
   const PrepRow *rows = a.rows + mi.row0;
-  const double pl = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.lo);
-  const double ph = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.hi);
+  const double pl = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.lo, mi.vs_half);
+  const double ph = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.hi, mi.vs_half);
   if (!(pl == pl) || !(ph == ph) || ph > pl) {
     ts.flags |= kFlagLoveFallback;
     return;
@@ -968,7 +971,7 @@ __global__ __launch_bounds__(kBlock) void love_count_kernel(PipeArgs a) {
   if (ok && want == K && exact > K) {
     // Capped: the K-th bracket must end below the (K+1)-th phase crossing.
     const Bracket b = a.brackets[static_cast<size_t>(tl) * a.kmax + K - 1];
-    const double pb = love_phase_dev(rows, mi.L, mi.top, ti.omega, b.hi);
+    const double pb = love_phase_dev(rows, mi.L, mi.top, ti.omega, b.hi, mi.vs_half);
     ok = pb == pb && love_roots_between(pl, pb) == K;
   }
   if (ok) {
@@ -993,7 +996,7 @@ __global__ __launch_bounds__(kBlock) void love_phase_kernel(PipeArgs a) {
     if (m >= mi.K) continue;
     const PrepRow *rows = a.rows + mi.row0;
     double lo = mi.lo, hi = mi.hi;
-    double pa = love_phase_dev(rows, mi.L, mi.top, ti.omega, lo), pb = love_phase_dev(rows, mi.L, mi.top, ti.omega, hi);
+    double pa = love_phase_dev(rows, mi.L, mi.top, ti.omega, lo, mi.vs_half), pb = love_phase_dev(rows, mi.L, mi.top, ti.omega, hi, mi.vs_half);
     const int exact = love_roots_between(pa, pb), want = exact < mi.K ? exact : mi.K;
     if (m == 0) a.scan[tl].count = want;
     if (m >= want) continue;
@@ -1009,7 +1012,7 @@ __global__ __launch_bounds__(kBlock) void love_phase_kernel(PipeArgs a) {
       }
       const double mid = lo + .5 * (hi - lo);
       if (!(mid > lo && mid < hi)) break;
-      const double p = love_phase_dev(rows, mi.L, mi.top, ti.omega, mid);
+      const double p = love_phase_dev(rows, mi.L, mi.top, ti.omega, mid, mi.vs_half);
       if (!(p == p)) break;
       if (p > level) { lo = mid; pa = p; } else { hi = mid; pb = p; }
     }
@@ -1282,36 +1285,42 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(PipeArgs a) {
   if (t >= a.trace_end) return;
   const TraceInfo ti = a.traces[t];
   const ModelInfo mi = a.models[ti.model];
-  const TraceScan ts = a.scan[tl];
+  TraceScan &ts = a.scan[tl];
   const double *roots = a.roots + static_cast<size_t>(tl) * a.kmax;
-  unsigned flags = 0;
-  int count = ts.count, level = a.fp32 ? kLevelScan32 : kLevelScan64;
+  unsigned flags = 0, rejected = 0;
+  int count = ts.count, level = a.level_grid;
+  const bool phase = (ts.flags & kScanLovePhase) != 0;
   if (ts.status) { level = ts.status == 1 ? -4 : -3; count = 0; }
-  if (ts.flags & kScanLovePhase) level = kLevelLovePhase;
+  else flags |= a.trace_extra;
+  if (phase) level = kLevelLovePhase;
   flags |= ts.flags & kFlagLoveFallback;
   if (ts.flags & kTraceOverflow) { flags |= kFlagUnconverged; level = -1; count = -1; }
   bool cutoff = false, close = false;
-  int nfinite = 0;
   for (int m = 0; m < count && m < mi.K; ++m) {
     const double r = roots[m];
     if (!(r == r)) { flags |= kFlagRefineFailed; continue; }
-    ++nfinite;
     if (m > 0 && roots[m - 1] == roots[m - 1] && r - roots[m - 1] <= a.close_rtol * r) close = true;
   }
   for (int m = 0; m < mi.modes; ++m) {
     double r = m < count ? roots[m] : NAN;
     if (r == r && r >= .99 * mi.vs_half) cutoff = true;
+    // Optional acceptance test of each reported root on |F| there (reference
+    // cpp/cpu/solver.cpp:333): a root that fails it is reported as NaN.
+    if (r == r && a.f_accept < 1.) {
+      const double v = secular_rows(a.rows + mi.row0, mi.L, mi.top, a.wave, ti.omega, r);
+      if (!(fabs(v) <= a.f_accept)) { r = NAN; flags |= kFlagRejected; rejected |= 1u << m; }
+    }
     const size_t cell = static_cast<size_t>(mi.out0) + static_cast<size_t>(m) * mi.T + ti.f;
     if (a.velocity64) a.velocity64[cell] = r;
     if (a.velocity32) a.velocity32[cell] = float(r);
   }
-  (void)nfinite;
   if (a.all_roots) {
     double *dst = a.all_roots + static_cast<size_t>(t) * a.kmax;
     for (int m = 0; m < a.kmax; ++m) dst[m] = m < count ? roots[m] : NAN;
   }
+  ts.rejected = rejected;
   if (cutoff) flags |= kFlagNearCutoff;
-  if (close) flags |= kFlagCloseRoot;
+  if (close && (a.close_grid || phase)) flags |= kFlagCloseRoot;
   const size_t tc = static_cast<size_t>(mi.trace_out0) + ti.f;
   a.flags[tc] = flags;
   a.levels[tc] = level;
@@ -1372,7 +1381,9 @@ __device__ __forceinline__ GD rescale(GD *e) {  // the scale is held constant
   return GD(1.);
 }
 
-__device__ __forceinline__ double group_at(const PrepRow *rows, int L, int top, int wave, double omega, double c) {
+// dF/dk (fixed omega) into *fk and dF/domega (fixed k) into *fw at (omega, c).
+__device__ __forceinline__ void secular_partials(const PrepRow *rows, int L, int top, int wave, double omega,
+                                                 double c, double *fk, double *fw) {
   const double iw = 1. / omega, iw2 = iw * iw;
   const GD k(omega / c, 1., 0.), w(omega, 0., 1.);
   auto layer = [&](const PrepRow &r) {
@@ -1405,7 +1416,57 @@ __device__ __forceinline__ double group_at(const PrepRow *rows, int L, int top, 
     F = e[0];
   }
   (void)w;
-  return -F.a / F.b;
+  *fk = F.a;
+  *fw = F.b;
+}
+
+__device__ __forceinline__ double group_at(const PrepRow *rows, int L, int top, int wave, double omega, double c) {
+  double fk, fw;
+  secular_partials(rows, L, top, wave, omega, c, &fk, &fw);
+  return -fk / fw;
+}
+
+// Group velocity of reported root m of a trace with the reference's flag
+// contract (cpp/common/group_velocity.hpp:9-12, 26-104; window and close test
+// cpp/cpu/solver.cpp:96-103). In its terms A = dF/dln c = -k F_k and
+// B = dF/dln omega = k F_k + omega F_w, U = c A / (A + B) = -F_k / F_w:
+//   MISSING_ROOT   no root;
+//   CLOSE_ROOT     a neighbouring root within close_rtol;
+//   POLISH_FAILED  the root is not strictly inside its window (the
+//                  neighbour midpoints, else the search window);
+//   ILL_CONDITIONED |A + B| <= 64 eps (|A| + |B|);
+//   DERIVATIVE_UNSTABLE the first-order rounding bound of U,
+//                  c (|B| dA + |A| dB) / (A + B)^2 with dA = g |A|, dB = g |B|
+//                  and g = 32 eps (L + 1) for the forward-mode partials,
+//                  exceeds 0.25 group_rtol max(c, |U|) (the reference's
+//                  acceptance test on its finite-difference error estimate);
+//   NONFINITE      a non-finite derivative or U.
+// A flagged root has U = NaN.
+__device__ __forceinline__ double group_result(const PrepRow *rows, int L, int top, int wave, double omega,
+                                               const double *roots, int m, int count, double lower, double upper,
+                                               double close_rtol, double group_rtol, unsigned *flags) {
+  constexpr double eps = 2.220446049250313e-16;
+  const double c = m < count ? roots[m] : NAN;
+  *flags = 0u;
+  if (!(c == c) || !(c > 0.)) { *flags = kGroupMissing; return NAN; }
+  const bool prev = m > 0, next = m + 1 < count;
+  if ((prev && c - roots[m - 1] <= close_rtol * c) || (next && roots[m + 1] - c <= close_rtol * roots[m + 1])) {
+    *flags = kGroupClose;
+    return NAN;
+  }
+  const double lo = prev ? .5 * (roots[m - 1] + c) : lower, hi = next ? .5 * (c + roots[m + 1]) : upper;
+  if (!(lo > 0. && c > lo && c < hi)) { *flags = kGroupPolish; return NAN; }
+  double fk, fw;
+  secular_partials(rows, L, top, wave, omega, c, &fk, &fw);
+  const double k = omega / c, A = -k * fk, B = k * fk + omega * fw, den = A + B;
+  if (!isfinite(A) || !isfinite(B)) { *flags = kGroupNonfinite; return NAN; }
+  if (!(fabs(den) > 64. * eps * (fabs(A) + fabs(B)))) { *flags = kGroupIll; return NAN; }
+  const double u = -fk / fw;
+  if (!isfinite(u)) { *flags = kGroupNonfinite; return NAN; }
+  const double g = 32. * eps * double(L + 1);
+  const double err = c * (2. * g * fabs(A) * fabs(B)) / (den * den);
+  if (!(err <= .25 * group_rtol * fmax(c, fabs(u)))) { *flags = kGroupUnstable; return NAN; }
+  return u;
 }
 
 // Rayleigh (0) and Love (1) get their own register allocations; two waves per
@@ -1426,19 +1487,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const TraceScan &ts = a.scan[tl];
     const size_t cell = static_cast<size_t>(mi.out0) + static_cast<size_t>(m) * mi.T + ti.f;
     const double *roots = a.roots + static_cast<size_t>(tl) * a.kmax;
-    const double c = m < ts.count ? roots[m] : NAN;
+    unsigned gf = kGroupMissing;
     double u = NAN;
-    unsigned gf = 0;
-    if (!(c == c)) gf = kGroupMissing;
-    else {
-      const bool close = (m > 0 && c - roots[m - 1] <= a.close_rtol * c) ||
-                         (m + 1 < ts.count && roots[m + 1] - c <= a.close_rtol * roots[m + 1]);
-      if (close) gf = kGroupClose;
-      else {
-        u = group_at(a.rows + mi.row0, mi.L, mi.top, Wave, ti.omega, c);
-        if (!(u == u) || fabs(u) == INFINITY) { u = NAN; gf = kGroupNonfinite; }
-      }
-    }
+    if (!((ts.rejected >> m) & 1u))
+      u = group_result(a.rows + mi.row0, mi.L, mi.top, Wave, ti.omega, roots, m, ts.count, mi.lo, mi.hi,
+                       a.close_rtol, a.group_rtol, &gf);
     a.group[cell] = u;
     a.group_flags[cell] = gf;
   }
@@ -1539,10 +1592,11 @@ __device__ bool cont_hidden_pair(const PrepRow *rows, int L, int top, int wave, 
 // phase levels at the bracket ends; Rayleigh by three samples on each open
 // side of the root, with a pair search at a same-sign trough.
 __device__ unsigned cont_window(const PrepRow *rows, int L, int top, int wave, double omega, double a, double b,
-                                double fa, double fb, double root, double tol, double level) {
+                                double fa, double fb, double root, double tol, double level, double vs_half) {
   if (wave) {
     constexpr double pi = 3.141592653589793;
-    const double left = love_phase_dev(rows, L, top, omega, a), right = love_phase_dev(rows, L, top, omega, b);
+    const double left = love_phase_dev(rows, L, top, omega, a, vs_half),
+                 right = love_phase_dev(rows, L, top, omega, b, vs_half);
     if (!(left == left) || !(right == right)) return kPredAuditFailed;
     if (floor(left / pi) != level || floor(right / pi) != level - 1.) return kPredAuditFailed | kPredReview;
     return 0u;
@@ -1643,7 +1697,7 @@ __global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
       // (continuation.hpp:170-212), refined in FP64, then audited.
       double first_level = 0.;
       if (a.wave) {
-        const double ph = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.lo);
+        const double ph = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.lo, mi.vs_half);
         if (!(ph == ph)) reason = kPredAuditFailed;
         first_level = floor(ph / 3.141592653589793);
       }
@@ -1667,8 +1721,12 @@ __global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
                                           ev);
         if (!(root >= x0 && root <= x1)) { reason = kPredCorrection; break; }
         if (j && root - roots[j - 1] <= a.close_rtol * fabs(root)) { reason = kPredCorrection | kPredReview; break; }
+        if (a.f_accept < 1.) {
+          const double fr = secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, root);
+          if (!(fabs(fr) <= a.f_accept)) { reason = kPredCorrection; break; }
+        }
         const unsigned issue = cont_window(rows, mi.L, mi.top, a.wave, ti.omega, x0, x1, f0, f1, root, a.tol,
-                                           first_level - j);
+                                           first_level - j, mi.vs_half);
         if (issue) { reason = issue; pred |= kPredAudit; break; }
         lo[j] = x0;
         hi[j] = x1;
@@ -1692,13 +1750,9 @@ __global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
       const size_t cell = static_cast<size_t>(mi.out0) + static_cast<size_t>(m) * mi.T + ti.f;
       a.velocity[cell] = c;
       if (a.group) {
-        double u = NAN;
         unsigned gf = 0;
-        if (!(c == c)) gf = kGroupMissing;
-        else {
-          u = group_at(rows, mi.L, mi.top, a.wave, ti.omega, c);
-          if (!(u == u) || fabs(u) == INFINITY) { u = NAN; gf = kGroupNonfinite; }
-        }
+        const double u = group_result(rows, mi.L, mi.top, a.wave, ti.omega, roots, m, nr, mi.lo, mi.hi, a.close_rtol,
+                                      a.group_rtol, &gf);
         a.group[cell] = u;
         a.group_flags[cell] = gf;
       }
@@ -1878,8 +1932,10 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
   mark(3);
   hipLaunchKernelGGL(assemble_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   if (pa.wave == 1) {
-    hipLaunchKernelGGL(love_count_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
-    hipLaunchKernelGGL(love_phase_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+    if (pa.love_phase) {
+      hipLaunchKernelGGL(love_count_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
+      hipLaunchKernelGGL(love_phase_kernel, dim3(pb), dim3(kBlock), 0, s, pa);
+    }
   }
   mark(4);
   if (pa.wave == 1) launch_refine<1>(pa, pb, s);

@@ -35,13 +35,19 @@ constexpr float kDipStopDrop = 1.5f;  // ... unless the log2 scale fell this far
 enum EventKind : int { kNone = 0, kBracket = 1, kUncertain = 2, kTrough = 3, kDip = 4, kProbe = 5 };
 
 // Trace-flag bits (reference python/modisp/__init__.py:15-32).
-constexpr unsigned kFlagUnconverged = 4u, kFlagNearCutoff = 16u, kFlagRefineFailed = 64u,
+constexpr unsigned kFlagUnconverged = 4u, kFlagNearCutoff = 16u, kFlagRefineFailed = 64u, kFlagRejected = 128u,
                    kFlagCloseRoot = 4096u, kFlagLoveFallback = 16384u;
 constexpr unsigned kTraceOverflow = 1u << 31;  // internal: event list overflowed
 // Group-velocity flags (reference cpp/common/group_velocity.hpp:9-12).
-constexpr unsigned kGroupMissing = 1u, kGroupClose = 16u, kGroupNonfinite = 32u;
-// search_level codes of this backend's grid scans.
-constexpr int kLevelScan32 = 20, kLevelScan64 = 21, kLevelLovePhase = 11;
+constexpr unsigned kGroupMissing = 1u, kGroupPolish = 2u, kGroupUnstable = 4u, kGroupIll = 8u, kGroupClose = 16u,
+                   kGroupNonfinite = 32u;
+// search_level codes (reference docs/API.md:203-209): a trace accepted by the
+// grid scan reports the code of the reference search its strategy names
+// (8/9/10 shape acceptance at high, balanced or auto, fast; 3 high adaptive;
+// 6/7 balanced/fast observation grid); Love traces whose roots follow the
+// exact phase count report 11.
+constexpr int kLevelShapeHigh = 8, kLevelShapeBalanced = 9, kLevelShapeFast = 10, kLevelAdaptive = 3,
+              kLevelObservationBalanced = 6, kLevelObservationFast = 7, kLevelLovePhase = 11;
 constexpr unsigned kScanLovePhase = 1u << 30;  // TraceScan: Love roots follow the exact phase count
 
 // One prepared layer row: reciprocals so a trace's constants are products.
@@ -112,6 +118,7 @@ struct TraceScan {
   int count;    // roots found (<= K), -1 unresolved
   int nodes;    // secular evaluations of the trace's scan (lane slots, batch overlaps included)
   unsigned umask, pmask;  // events (bit e) that are FP32-uncertain / probed for a hidden pair
+  unsigned rejected;      // reported modes (bit m) whose |F| failed the f_accept threshold
 };
 
 // A root bracket with F at its ends and the summed log2 normalization scales
@@ -155,8 +162,15 @@ struct PipeArgs {
   int wave;
   int fp32;
   int refinement;  // 0 legacy regula falsi, 1 Brent-Dekker, 2 Chandrupatla
+  int level_grid;  // search_level of a trace the grid scan accepts (kLevelShape*, kLevelAdaptive, ...)
+  int close_grid;  // 1: close roots of a scan-accepted trace raise kFlagCloseRoot (0 for high qe_adaptive
+                   // Rayleigh, whose search never sets it; Love phase traces always test)
+  int love_phase;  // 1: Love roots are verified by the exact phase count (reference love_phase=1)
+  unsigned trace_extra;  // flags every searched trace carries (kFlagApproximate for approximate presets)
+  double f_accept;  // reported roots with |F| > f_accept are rejected (reference solver.cpp:333); 1 = off
   double tol;
   double close_rtol;
+  double group_rtol;  // group velocities with a larger rounding bound are flagged unstable
   double *velocity64;
   float *velocity32;
   uint32_t *flags;
@@ -202,6 +216,8 @@ struct ContArgs {
   const double *anchor_roots;  // [anchor][kmax]
   double *anchor_slopes;       // [anchor][kmax]
   double tol, close_rtol;
+  double f_accept;             // a continued root with |F| > f_accept falls back (continuation.hpp:201-204)
+  double group_rtol;
   double *velocity;
   uint32_t *flags;
   int32_t *levels, *counts;

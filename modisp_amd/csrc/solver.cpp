@@ -83,6 +83,7 @@ struct md_rocm_solver {
   hipGraphExec_t graph_exec = nullptr;
   std::vector<uintptr_t> graph_key;  // arguments the captured graph was built for
   long long cont_accepted = 0, cont_fallbacks = 0;  // continuation targets of the last run
+  double f_accept = 1.;            // acceptance threshold of the current call (1 = off)
   int prep_models = 0;             // models prepared on the device for the current run
   int prep_rows = 0;               // largest row count among them
   DevBuf table, ntab;              // per-model scan tables (adaptive grid)
@@ -104,6 +105,29 @@ struct md_rocm_solver {
 };
 
 namespace {
+
+// search_level of a trace the grid scan accepts: the code of the reference
+// search its (accuracy, shape) pair selects (docs/API.md:203-209; levels set at
+// cpp/cpu/solver.cpp:300 and :307, the adaptive ladder at :258-283).
+int grid_level(const md_rocm_config &c) {
+  const bool fast = c.accuracy == 2, approximate = c.accuracy != 0;
+  if (c.shape) return !approximate ? mdr::kLevelShapeHigh : fast ? mdr::kLevelShapeFast : mdr::kLevelShapeBalanced;
+  return !approximate ? mdr::kLevelAdaptive : fast ? mdr::kLevelObservationFast : mdr::kLevelObservationBalanced;
+}
+
+// The f_accept of an md_rocm_run call, held by the solver for the call's
+// duration (the device entry points run with the test off).
+struct AcceptScope {
+  md_rocm_solver *s;
+  AcceptScope(md_rocm_solver *solver, double accept) : s(solver) { s->f_accept = accept; }
+  ~AcceptScope() { s->f_accept = 1.; }
+};
+
+uintptr_t accept_bits(const md_rocm_solver *s) {
+  uintptr_t b;
+  std::memcpy(&b, &s->f_accept, sizeof(b));
+  return b;
+}
 
 int chunk_size(const md_rocm_solver *s, long long ntraces) {
   long long c = s->cfg.chunk_traces > 0 ? s->cfg.chunk_traces : (1LL << 20);
@@ -171,8 +195,14 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   a.wave = wave;
   a.fp32 = s->cfg.single_precision ? 1 : 0;
   a.refinement = s->cfg.refinement;
+  a.level_grid = grid_level(s->cfg);
+  a.close_grid = s->cfg.shape || s->cfg.accuracy != 0 ? 1 : 0;
+  a.love_phase = s->cfg.love_phase;
+  a.trace_extra = s->cfg.accuracy != 0 ? mdr::kFlagApproximate : 0u;
+  a.f_accept = s->f_accept;
   a.tol = tol;
   a.close_rtol = s->cfg.close_root_rtol;
+  a.group_rtol = s->cfg.group_rtol;
   a.velocity64 = v64;
   a.velocity32 = v32;
   a.flags = flags;
@@ -270,6 +300,8 @@ int run_continuation(md_rocm_solver *s, int kmax, int max_rows, int wave, double
   c.anchor_slopes = s->c_slopes.as<double>();
   c.tol = tol;
   c.close_rtol = s->cfg.close_root_rtol;
+  c.f_accept = s->f_accept;
+  c.group_rtol = s->cfg.group_rtol;
   c.velocity = v64;
   c.flags = flags;
   c.levels = levels;
@@ -309,6 +341,18 @@ int check_config(const md_rocm_config &c) {
     return fail(2, "scan_waves must be 0, 1, 2, 4 or 8");
   if (c.consumer_blocks < 0 || c.consumer_blocks > 64) return fail(2, "consumer_blocks must be 0..64");
   if (c.refinement < 0 || c.refinement > 2) return fail(2, "refinement must be 0 (legacy), 1 (brent) or 2 (chandrupatla)");
+  if (c.accuracy != 0 && c.accuracy != 1 && c.accuracy != 2 && c.accuracy != 4)
+    return fail(2, "accuracy must be 0 (high), 1 (balanced), 2 (fast) or 4 (auto); 3 (uniform) is the legacy "
+                   "path: set grid=0");
+  if (c.shape != 0 && c.shape != 1) return fail(2, "shape must be 0 or 1");
+  if (c.love_phase != 0 && c.love_phase != 1) return fail(2, "invalid refinement or Love strategy");
+  if (c.secular != 1)
+    return fail(2, "secular must be 1 (fast): the released 25-entry Dunkin evaluator (secular=0) is not built on "
+                   "this backend");
+  if (!(c.sampling_rtol == -1. || (c.sampling_rtol > 0. && c.sampling_rtol <= 1.)))
+    return fail(2, "sampling_rtol must be finite and between 0 and 1 (or -1 for automatic sampling)");
+  if (!(c.period_threshold >= 0.) || !std::isfinite(c.period_threshold))
+    return fail(2, "period_threshold must be finite and non-negative");
   return 0;
 }
 
@@ -333,6 +377,11 @@ void md_rocm_default_config(md_rocm_config *cfg) {
   cfg->grid = 1;
   cfg->rho = 8.;
   cfg->refinement = 1;
+  cfg->shape = 1;
+  cfg->love_phase = 1;
+  cfg->secular = 1;
+  cfg->sampling_rtol = 1e-3;
+  cfg->period_threshold = 0.;
 }
 
 const char *md_rocm_last_error(void) { return g_error.c_str(); }
@@ -547,7 +596,7 @@ int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double 
       static_cast<uintptr_t>(row_periods), reinterpret_cast<uintptr_t>(bounds), tb, static_cast<uintptr_t>(wave),
       reinterpret_cast<uintptr_t>(velocity), reinterpret_cast<uintptr_t>(flags), reinterpret_cast<uintptr_t>(levels),
       reinterpret_cast<uintptr_t>(counts), reinterpret_cast<uintptr_t>(group), reinterpret_cast<uintptr_t>(group_flags),
-      static_cast<uintptr_t>(s->cfg.scan_waves), static_cast<uintptr_t>(s->cfg.consumer_blocks)};
+      static_cast<uintptr_t>(s->cfg.scan_waves), static_cast<uintptr_t>(s->cfg.consumer_blocks), accept_bits(s)};
   if (s->cfg.graph && s->graph_exec && key == s->graph_key) {
     MD_HIP(hipEventRecord(s->ev_begin, s->stream));
     MD_HIP(hipGraphLaunch(s->graph_exec, s->stream));
@@ -636,10 +685,10 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
   if (B < 1 || B > s->capacity) return fail(2, "B must be 1..capacity");
   if (!thickness || !vp || !vs || !rho || !periods || !bounds || !velocity || !flags || !levels || !counts)
     return fail(2, "null input or output pointer");
-  if (!(f_accept >= 0.)) return fail(2, "f_accept must be non-negative");
-  // |F| <= 1 always holds for this backend's normalised secular function; a
-  // tighter acceptance threshold would need the reference's normalisation.
-  if (f_accept < 1.) return fail(2, "f_accept < 1 is not supported by the ROCm backend");
+  if (!(f_accept >= 0.) || !std::isfinite(f_accept)) return fail(2, "f_accept must be non-negative");
+  // Reported roots with |F| > f_accept (the max-normalised secular function,
+  // as the reference's) become NaN with TRACE_F_REJECTED; 1 disables the test.
+  AcceptScope accept_scope(s, f_accept);
   const int L = s->L, T = s->nf, M = s->nmodes;
   const size_t nrow = static_cast<size_t>(B) * L, ncell = static_cast<size_t>(B) * M * T,
                ntr = static_cast<size_t>(B) * T;

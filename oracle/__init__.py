@@ -72,6 +72,9 @@ def lib():
         L.mdo_refine.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _ip]
         L.mdo_refine.restype = ctypes.c_double
+        L.mdo_love_phase_roots.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_double, _dp, _ip]
+        L.mdo_love_phase_roots.restype = ctypes.c_int
         L.mdo_group_fd.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                    ctypes.POINTER(ctypes.c_uint)]
@@ -134,6 +137,17 @@ def secular(h, vp, vs, rho, period, c, wave=0, shape=False):
 def love_phase(h, vp, vs, rho, period, c):
     h, vp, vs, rho = (_f64(x).ravel() for x in (h, vp, vs, rho))
     return lib().mdo_love_phase(_p(h), _p(vp), _p(vs), _p(rho), h.size, float(period), float(c))
+
+
+def love_phase_roots(h, vp, vs, rho, period, low, high, K, tol):
+    """Love roots by the unwrapped SH phase: (ok, roots); ok False where the
+    reference's phase path fails and it falls back to its scan."""
+    h, vp, vs, rho = (_f64(x).ravel() for x in (h, vp, vs, rho))
+    roots = np.empty(int(K))
+    nr = ctypes.c_int(0)
+    ok = lib().mdo_love_phase_roots(_p(h), _p(vp), _p(vs), _p(rho), h.size, float(period), float(low), float(high),
+                                    int(K), float(tol), _p(roots), ctypes.byref(nr))
+    return bool(ok), roots[:nr.value].copy()
 
 
 def refine(h, vp, vs, rho, period, a, b, tol, wave=0, method=1):

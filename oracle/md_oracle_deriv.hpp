@@ -338,6 +338,39 @@ inline double group_velocity_analytic(const double *h, const double *vp, const d
   return -fk.d / fw.d;
 }
 
+// The analytic group velocity with the reference's flag contract
+// (cpp/common/group_velocity.hpp:9-12, 26-104), in its terms
+// A = dF/dln c = -k F_k, B = dF/dln omega = k F_k + omega F_w:
+// POLISH_FAILED (2) when the root is not strictly inside (lower, upper);
+// ILL_CONDITIONED (8) when |A + B| <= 64 eps (|A| + |B|); DERIVATIVE_UNSTABLE
+// (4) when the first-order rounding bound c (|B| dA + |A| dB) / (A + B)^2,
+// dA = g |A|, dB = g |B|, g = 32 eps (L + 1), exceeds 0.25 rtol max(c, |U|);
+// NONFINITE (32). A flagged root has U = NaN. The bound and thresholds are
+// the device kernel's, so that the two agree flag for flag.
+inline double group_result_analytic(const double *h, const double *vp, const double *vs, const double *rho, int L,
+                                    int wave, double omega, double c, double lower, double upper, double rtol,
+                                    unsigned &flags) {
+  constexpr double eps = 2.220446049250313e-16;
+  flags = 0u;
+  if (!(lower > 0. && c > lower && c < upper)) { flags = 2u; return NAN; }
+  Dual fk = secular_direction(h, vp, vs, rho, L, wave, omega, c, 0);
+  const double k0 = omega / c, kn = k0 - fk.v / fk.d, cn = omega / kn;
+  if (std::isfinite(cn) && std::fabs(cn - c) <= 1e-10 * c && cn != c) {
+    c = cn;
+    fk = secular_direction(h, vp, vs, rho, L, wave, omega, c, 0);
+  }
+  const Dual fw = secular_direction(h, vp, vs, rho, L, wave, omega, c, 1);
+  const double k = omega / c, A = -k * fk.d, B = k * fk.d + omega * fw.d, den = A + B;
+  if (!std::isfinite(A) || !std::isfinite(B)) { flags = 32u; return NAN; }
+  if (!(std::fabs(den) > 64. * eps * (std::fabs(A) + std::fabs(B)))) { flags = 8u; return NAN; }
+  const double u = -fk.d / fw.d;
+  if (!std::isfinite(u)) { flags = 32u; return NAN; }
+  const double g = 32. * eps * double(L + 1);
+  const double err = c * (2. * g * std::fabs(A) * std::fabs(B)) / (den * den);
+  if (!(err <= .25 * rtol * std::fmax(c, std::fabs(u)))) { flags = 4u; return NAN; }
+  return u;
+}
+
 // Implicit phase-velocity gradient dc/dm = -(dF/dm)/(dF/dc) at a root, for
 // every layer parameter. out[4*j + {0,1,2,3}] = dc/d{h,vp,vs,rho}[j].
 inline void phase_gradient(const double *h, const double *vp, const double *vs, const double *rho, int L, int wave,

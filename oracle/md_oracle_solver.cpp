@@ -363,10 +363,10 @@ void solve_trace(const Model &model, double lo, double hi, double period, const 
         out.group[cell] = g.velocity;
         out.group_flags[cell] = g.flags;
       } else {
-        const double u = group_velocity_analytic(model.view.h, model.view.vp, model.view.vs, model.view.rho, L,
-                                                 cfg.wave, omega, root);
-        out.group[cell] = finite(u) ? u : NAN;
-        out.group_flags[cell] = finite(u) ? 0u : static_cast<unsigned>(GROUP_NONFINITE);
+        unsigned gf = 0;
+        out.group[cell] = group_result_analytic(model.view.h, model.view.vp, model.view.vs, model.view.rho, L,
+                                                cfg.wave, omega, root, glo, ghi, cfg.group_rtol, gf);
+        out.group_flags[cell] = gf;
       }
     }
   }
@@ -430,6 +430,17 @@ double mdo_love_phase(const double *h, const double *vp, const double *vs, const
                       double c) {
   const Medium m = plain_medium(h, vp, vs, rho, L, 1);
   return love_phase(m, two_pi() / period, c);
+}
+
+// The Love roots by the unwrapped phase (reference love_phase.hpp:76-91):
+// returns 1 on success (nr roots), 0 when the reference would fall back.
+int mdo_love_phase_roots(const double *h, const double *vp, const double *vs, const double *rho, int L,
+                         double period, double low, double high, int K, double tol, double *roots, int *nr) {
+  const Medium m = plain_medium(h, vp, vs, rho, L, 1);
+  int n = 0, ev = 0;
+  const bool ok = love_phase_roots(m, two_pi() / period, low, high, K, tol, roots, n, ev);
+  *nr = n;
+  return ok ? 1 : 0;
 }
 
 // One bracket refinement; method 0 legacy, 1 Brent, 2 Chandrupatla.

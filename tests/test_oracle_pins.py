@@ -285,10 +285,14 @@ def test_oracle_matches_reference_headers():
                 roots = roots[np.isfinite(roots)]
                 for j, r in enumerate(roots):
                     a_, b_ = r * (1 - 2e-3), r * (1 + 2e-3)
-                    mine, _ = oracle.refine(h, vp, vs, rho, T, a_, b_, 1e-12, wave, 1)
-                    ref = R.mdr_refine(P(h), P(vp), P(vs), P(rho), h.size, wave, ctypes.c_double(T),
-                                       ctypes.c_double(a_), ctypes.c_double(b_), ctypes.c_double(1e-12), 1, None)
-                    assert mine == ref or (np.isnan(mine) and np.isnan(ref))
+                    for method in (0, 1, 2):   # legacy, Brent, Chandrupatla (refinement.hpp:11-65)
+                        mine, ev_m = oracle.refine(h, vp, vs, rho, T, a_, b_, 1e-12, wave, method)
+                        ev_r = ctypes.c_int(0)
+                        ref = R.mdr_refine(P(h), P(vp), P(vs), P(rho), h.size, wave, ctypes.c_double(T),
+                                           ctypes.c_double(a_), ctypes.c_double(b_), ctypes.c_double(1e-12), method,
+                                           ctypes.byref(ev_r))
+                        assert mine == ref or (np.isnan(mine) and np.isnan(ref)), (method, mine, ref)
+                        assert ev_m == ev_r.value, (method, ev_m, ev_r.value)
                     lo = b[0] if j == 0 else .5 * (roots[j - 1] + r)
                     hi = b[1] if j + 1 == roots.size else .5 * (r + roots[j + 1])
                     fl = ctypes.c_uint(0)
@@ -297,6 +301,14 @@ def test_oracle_matches_reference_headers():
                     um, flm = oracle.group_fd(h, vp, vs, rho, T, r, lo, hi, 1e-3, wave)
                     assert (um == u or (np.isnan(um) and np.isnan(u))) and flm == fl.value
                 if wave == 1:
+                    for K, tol in ((4, 1e-12), (7, 1e-6), (12, 1e-9)):   # love_phase.hpp:76-91
+                        ok, mine = oracle.love_phase_roots(h, vp, vs, rho, T, b[0], b[1], K, tol)
+                        rr, nr = np.empty(K), ctypes.c_int(0)
+                        rok = R.mdr_love_phase_roots(P(h), P(vp), P(vs), P(rho), h.size, ctypes.c_double(T),
+                                                     ctypes.c_double(b[0]), ctypes.c_double(b[1]), K,
+                                                     ctypes.c_double(tol), P(rr), ctypes.byref(nr))
+                        assert ok == bool(rok) and mine.size == nr.value
+                        np.testing.assert_array_equal(mine.view(np.uint64), rr[:nr.value].view(np.uint64))
                     for cc in np.linspace(b[0], b[1], 7)[1:-1]:
                         mine = oracle.love_phase(h, vp, vs, rho, T, cc)
                         ref = R.mdr_love_phase(P(h), P(vp), P(vs), P(rho), h.size, ctypes.c_double(T), ctypes.c_double(cc))
