@@ -121,6 +121,31 @@ int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double 
                        double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
                        int32_t *counts, double *group, uint32_t *group_flags);
 
+/* Bytes of the fixed device workspace of md_rocm_run_device_static, per GPU. */
+#define MD_ROCM_STATIC_SCRATCH_BYTES (128u << 20)
+
+/* As md_rocm_run_device (same arguments, results bit for bit the same), but
+ * it allocates nothing: no hipMalloc, hipFree or allocating library call.
+ * Every work buffer (prepared rows, models, traces, scan tables, events,
+ * brackets, queues, counters) lives in one fixed device array of
+ * MD_ROCM_STATIC_SCRATCH_BYTES per GPU, set up by md_rocm_create and held
+ * until the process ends; a batch is processed in as many trace chunks as
+ * that array needs (about 3.4 KB of chunk buffers per trace; per model its
+ * prepared rows, scan table and 24 B per period stay resident, 4.7 KB for 13
+ * rows and 100 periods). A batch whose prepared inputs alone do not fit
+ * (above about 28,000 such models) is rejected with code 2 before any launch.
+ * Every call on one GPU shares that GPU's array and leaves its contents
+ * behind (counters are cleared by a kernel at the start of each chunk), so
+ * calls that overlap on one GPU, from any solver or thread, race: serialise
+ * them on one stream. Asynchronous on the solver's stream; no stage timing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). */
+int md_rocm_run_device_static(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                              const double *rho, int B, const double *periods, int row_periods,
+                              const double *bounds, double c_tol_rel, int wave, double *velocity, uint32_t *flags,
+                              int32_t *levels, int32_t *counts, double *group, uint32_t *group_flags);
+
 /* Adjoint of md_rocm_run_device's roots (all pointers DEVICE memory, [B, L]
  * models, [nf] or [B, nf] periods, [B, nmodes, nf] roots and weights):
  * grad_p[b, j] = sum over modes and periods of weight * dc/dp_j for

@@ -367,6 +367,17 @@ class QeSolver:
             p.get("group"), p.get("group_flags"))
         _native.check(code)
 
+    def run_device_static_batch(self, B, pointers, c_tol_rel, wave, row_periods=False):
+        """run_device_batch through md_rocm_run_device_static: no allocation,
+        every work buffer in the library's fixed per-GPU device array; calls
+        that overlap on one GPU race."""
+        p = pointers
+        code = _native.lib().md_rocm_run_device_static(
+            self.handle, p["thickness"], p["vp"], p["vs"], p["rho"], int(B), p["periods"], int(bool(row_periods)),
+            p["bounds"], float(c_tol_rel), int(wave), p["velocity"], p["flags"], p["levels"], p["counts"],
+            p.get("group"), p.get("group_flags"))
+        _native.check(code)
+
     def autotune_device_batch(self, B, pointers, c_tol_rel, wave, row_periods=False):
         """Time the launch-shape candidates on this device batch (arguments as
         run_device_batch; outputs are overwritten) and keep the fastest.

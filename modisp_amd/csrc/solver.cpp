@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -63,6 +64,38 @@ constexpr int kStatCount = mdr::kStats;  // see md_rocm_last_stats
 
 }  // namespace
 
+// The fixed device workspace of md_rocm_run_device_static: one array per GPU,
+// held until the process ends. In host code `scratch` names only a host-side
+// stand-in; the device address of each GPU's copy comes from
+// hipGetSymbolAddress, called once per device with that device current
+// (scratch_setup, from md_rocm_create), and host code uses that address only.
+constexpr size_t kScratchBytes = size_t(MD_ROCM_STATIC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+int g_scratch_state[kMaxDevices] = {};  // 0 not set up, 1 address known, -1 lookup failed
+
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_state[device]) return;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_state[device] = e == hipSuccess && ptr ? 1 : -1;
+  g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+  if (e != hipSuccess) (void)hipGetLastError();
+}
+
+void *scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  return g_scratch_state[device] == 1 ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
 struct md_rocm_solver {
   md_rocm_config cfg{};
   int capacity = 0, L = 0, nf = 0, nmodes = 0;
@@ -84,6 +117,7 @@ struct md_rocm_solver {
   std::vector<uintptr_t> graph_key;  // arguments the captured graph was built for
   long long cont_accepted = 0, cont_fallbacks = 0;  // continuation targets of the last run
   double f_accept = 1.;            // acceptance threshold of the current call (1 = off)
+  bool timed = false;              // ev_begin / ev_end were recorded by the last run
   int prep_models = 0;             // models prepared on the device for the current run
   int prep_rows = 0;               // largest row count among them
   DevBuf table, ntab;              // per-model scan tables (adaptive grid)
@@ -136,10 +170,29 @@ int chunk_size(const md_rocm_solver *s, long long ntraces) {
   return static_cast<int>(c);
 }
 
-// Runs the root pipeline over all traces already prepared on the device.
-int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, int wave, double tol, double *v64,
-                 float *v32, uint32_t *flags, int32_t *levels, int32_t *counts, double *group, uint32_t *gflags,
-                 const mdr::TraceInfo *list = nullptr, double *all_roots = nullptr) {
+// Device buffers of one run: the prepared rows, models and traces, the
+// per-model scan tables, and the per-chunk work buffers of the pipeline.
+struct Work {
+  mdr::PrepRow *rows = nullptr;
+  mdr::ModelInfo *models = nullptr;
+  mdr::TraceInfo *traces = nullptr;
+  mdr::TablePoint *table = nullptr;
+  int *ntab = nullptr;
+  mdr::Event *events = nullptr;
+  mdr::TraceScan *scan = nullptr;
+  mdr::Bracket *brackets = nullptr;
+  double *roots = nullptr;
+  uint32_t *unc_q = nullptr, *tr_q = nullptr;
+  int *counters = nullptr, *overflow = nullptr, *phase = nullptr;
+  unsigned long long *stats = nullptr;
+  int chunk = 0;           // traces per chunk
+  hipEvent_t *marks = nullptr;  // stage events per chunk (nullptr: no stage timing)
+};
+
+size_t table_stride(int max_rows) { return mdr::kFloor + 2 * std::max(1, max_rows - 1); }
+
+// The solver's own buffers for a run of ntraces traces (grown on demand).
+int solver_work(md_rocm_solver *s, long long ntraces, int kmax, Work &w) {
   const int chunk = chunk_size(s, ntraces);
   const size_t qcap = static_cast<size_t>(chunk) * mdr::kEventCap;
   if (int e = s->events.ensure(qcap * sizeof(mdr::Event))) return e;
@@ -152,39 +205,71 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   if (int e = s->overflow.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->phase.ensure(static_cast<size_t>(chunk) * sizeof(int))) return e;
   if (int e = s->stats.ensure(kStatCount * sizeof(unsigned long long))) return e;
-  // Per-model scan tables for the adaptive grid (one wavefront per model).
-  const int stride = mdr::kFloor + 2 * std::max(1, s->prep_rows - 1);
   if (s->cfg.grid) {
-    if (int e = s->table.ensure(static_cast<size_t>(std::max(1, s->prep_models)) * stride * sizeof(mdr::TablePoint)))
-      return e;
-    if (int e = s->ntab.ensure(static_cast<size_t>(std::max(1, s->prep_models)) * sizeof(int))) return e;
-    MD_HIP(mdr::launch_tables(s->models.as<mdr::ModelInfo>(), s->rows.as<mdr::PrepRow>(), s->prep_models,
-                              s->prep_rows, wave, stride, s->table.as<mdr::TablePoint>(), s->ntab.as<int>(),
-                              s->stream));
+    const size_t stride = table_stride(s->prep_rows), nm = static_cast<size_t>(std::max(1, s->prep_models));
+    if (int e = s->table.ensure(nm * stride * sizeof(mdr::TablePoint))) return e;
+    if (int e = s->ntab.ensure(nm * sizeof(int))) return e;
   }
+  const int nchunks = static_cast<int>((ntraces + chunk - 1) / chunk);
+  while (static_cast<int>(s->marks.size()) < nchunks * mdr::kStageMarks) {
+    hipEvent_t e;
+    MD_HIP(hipEventCreate(&e));
+    s->marks.push_back(e);
+  }
+  w.rows = s->rows.as<mdr::PrepRow>();
+  w.models = s->models.as<mdr::ModelInfo>();
+  w.traces = s->traces.as<mdr::TraceInfo>();
+  w.table = s->table.as<mdr::TablePoint>();
+  w.ntab = s->ntab.as<int>();
+  w.events = s->events.as<mdr::Event>();
+  w.scan = s->scan.as<mdr::TraceScan>();
+  w.brackets = s->brackets.as<mdr::Bracket>();
+  w.roots = s->roots.as<double>();
+  w.unc_q = s->unc_q.as<uint32_t>();
+  w.tr_q = s->tr_q.as<uint32_t>();
+  w.counters = s->counters.as<int>();
+  w.overflow = s->overflow.as<int>();
+  w.phase = s->phase.as<int>();
+  w.stats = s->stats.as<unsigned long long>();
+  w.chunk = chunk;
+  w.marks = s->capturing ? nullptr : s->marks.data();
+  return 0;
+}
+
+// Runs the root pipeline over all traces already prepared on the device.
+int run_pipeline(md_rocm_solver *s, const Work &w, long long ntraces, int kmax, int max_rows, int wave, double tol,
+                 double *v64, float *v32, uint32_t *flags, int32_t *levels, int32_t *counts, double *group,
+                 uint32_t *gflags, const mdr::TraceInfo *list = nullptr, double *all_roots = nullptr) {
+  const int chunk = w.chunk;
+  const size_t qcap = static_cast<size_t>(chunk) * mdr::kEventCap;
+  // Per-model scan tables for the adaptive grid (one wavefront per model).
+  const int stride = static_cast<int>(table_stride(s->prep_rows));
+  if (s->cfg.grid)
+    MD_HIP(mdr::launch_tables(w.models, w.rows, s->prep_models, s->prep_rows, wave, stride, w.table, w.ntab,
+                              s->stream));
   mdr::PipeArgs a{};
-  a.table = s->table.as<mdr::TablePoint>();
-  a.ntab = s->ntab.as<int>();
+  a.table = w.table;
+  a.ntab = w.ntab;
   a.table_stride = stride;
-  a.traces = list ? list : s->traces.as<mdr::TraceInfo>();
+  a.traces = list ? list : w.traces;
   a.all_roots = all_roots;
-  a.models = s->models.as<mdr::ModelInfo>();
-  a.rows = s->rows.as<mdr::PrepRow>();
-  a.events = s->events.as<mdr::Event>();
-  a.scan = s->scan.as<mdr::TraceScan>();
-  a.brackets = s->brackets.as<mdr::Bracket>();
-  a.roots = s->roots.as<double>();
-  a.unc_queue = s->unc_q.as<uint32_t>();
-  a.trough_queue = s->tr_q.as<uint32_t>();
-  a.counters = s->counters.as<int>();
+  a.models = w.models;
+  a.rows = w.rows;
+  a.events = w.events;
+  a.scan = w.scan;
+  a.brackets = w.brackets;
+  a.roots = w.roots;
+  a.unc_queue = w.unc_q;
+  a.trough_queue = w.tr_q;
+  a.counters = w.counters;
   a.unc_count = a.counters;
   a.trough_count = a.counters + 1;
   a.overflow_count = a.counters + 2;
-  a.overflow_list = s->overflow.as<int>();
+  a.overflow_list = w.overflow;
   a.phase_count = a.counters + 3;
   a.handoff_count = a.counters + 4;
-  a.handoff_list = s->unc_q.as<uint32_t>();
-  a.phase_list = s->phase.as<int>();
+  a.handoff_list = w.unc_q;
+  a.phase_list = w.phase;
   a.queue_cap = static_cast<int>(qcap);
   a.nc = s->cfg.nc;
   a.max_rows = max_rows;
@@ -210,24 +295,74 @@ int run_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, i
   a.counts = counts;
   a.group = group;
   a.group_flags = gflags;
-  a.stats = s->stats.as<unsigned long long>();
+  a.stats = w.stats;
   const int persistent = s->cus * (s->cfg.consumer_blocks > 0 ? s->cfg.consumer_blocks : 8);
   const int nchunks = static_cast<int>((ntraces + chunk - 1) / chunk);
-  while (static_cast<int>(s->marks.size()) < nchunks * mdr::kStageMarks) {
-    hipEvent_t e;
-    MD_HIP(hipEventCreate(&e));
-    s->marks.push_back(e);
-  }
-  s->marked_chunks = s->capturing ? 0 : nchunks;
+  s->marked_chunks = w.marks ? nchunks : 0;
   int c = 0;
   for (long long begin = 0; begin < ntraces; begin += chunk, ++c) {
     a.trace_begin = static_cast<int>(begin);
     a.trace_end = static_cast<int>(std::min<long long>(ntraces, begin + chunk));
     const hipError_t err = mdr::launch_pipeline(a, a, a.fp32 != 0, group != nullptr, persistent, s->stream,
-                                                s->capturing ? nullptr : s->marks.data() + c * mdr::kStageMarks);
+                                                w.marks ? w.marks + c * mdr::kStageMarks : nullptr);
     if (err != hipSuccess) return fail(1, std::string("kernel launch: ") + hipGetErrorString(err));
   }
   return 0;
+}
+
+// run_pipeline on the solver's own buffers.
+int solver_pipeline(md_rocm_solver *s, long long ntraces, int kmax, int max_rows, int wave, double tol, double *v64,
+                    uint32_t *flags, int32_t *levels, int32_t *counts, double *group, uint32_t *gflags) {
+  Work w;
+  if (int e = solver_work(s, ntraces, kmax, w)) return e;
+  return run_pipeline(s, w, ntraces, kmax, max_rows, wave, tol, v64, nullptr, flags, levels, counts, group, gflags);
+}
+
+// Work buffers carved from the fixed device workspace at `base` (bytes long)
+// for B models of L rows and T periods: the prepared inputs and tables first,
+// then as many traces per chunk as the rest holds. Returns the bytes needed
+// for one trace per chunk when even that does not fit (0 on success).
+size_t carve_work(const md_rocm_solver *s, unsigned char *base, size_t bytes, int B, int L, int T, int kmax,
+                  Work &w) {
+  const auto up = [](size_t n) { return (n + 255) & ~size_t(255); };
+  const size_t nb = static_cast<size_t>(B), ntr = nb * T, stride = table_stride(L);
+  size_t fixed = up(nb * L * sizeof(mdr::PrepRow)) + up(nb * sizeof(mdr::ModelInfo)) +
+                 up(ntr * sizeof(mdr::TraceInfo)) + up(5 * sizeof(int)) + up(kStatCount * sizeof(unsigned long long));
+  if (s->cfg.grid) fixed += up(nb * stride * sizeof(mdr::TablePoint)) + up(nb * sizeof(int));
+  const size_t per_trace = mdr::kEventCap * (sizeof(mdr::Event) + 2 * sizeof(uint32_t)) + sizeof(mdr::TraceScan) +
+                           static_cast<size_t>(kmax) * (sizeof(mdr::Bracket) + sizeof(double)) + 2 * sizeof(int);
+  const size_t slack = 9 * 256;  // alignment of the nine per-chunk arrays
+  if (fixed + slack + per_trace > bytes) return fixed + slack + per_trace;
+  size_t chunk = (bytes - fixed - slack) / per_trace;
+  if (s->cfg.chunk_traces > 0) chunk = std::min(chunk, static_cast<size_t>(s->cfg.chunk_traces));
+  chunk = std::min(chunk, std::min(ntr, size_t(1) << 20));
+  size_t off = 0;
+  const auto take = [&](size_t n) {
+    unsigned char *p = base + off;
+    off += up(n);
+    return static_cast<void *>(p);
+  };
+  w.rows = static_cast<mdr::PrepRow *>(take(nb * L * sizeof(mdr::PrepRow)));
+  w.models = static_cast<mdr::ModelInfo *>(take(nb * sizeof(mdr::ModelInfo)));
+  w.traces = static_cast<mdr::TraceInfo *>(take(ntr * sizeof(mdr::TraceInfo)));
+  w.counters = static_cast<int *>(take(5 * sizeof(int)));
+  w.stats = static_cast<unsigned long long *>(take(kStatCount * sizeof(unsigned long long)));
+  if (s->cfg.grid) {
+    w.table = static_cast<mdr::TablePoint *>(take(nb * stride * sizeof(mdr::TablePoint)));
+    w.ntab = static_cast<int *>(take(nb * sizeof(int)));
+  }
+  const size_t qcap = chunk * mdr::kEventCap;
+  w.events = static_cast<mdr::Event *>(take(qcap * sizeof(mdr::Event)));
+  w.unc_q = static_cast<uint32_t *>(take(qcap * sizeof(uint32_t)));
+  w.tr_q = static_cast<uint32_t *>(take(qcap * sizeof(uint32_t)));
+  w.scan = static_cast<mdr::TraceScan *>(take(chunk * sizeof(mdr::TraceScan)));
+  w.brackets = static_cast<mdr::Bracket *>(take(chunk * kmax * sizeof(mdr::Bracket)));
+  w.roots = static_cast<double *>(take(chunk * kmax * sizeof(double)));
+  w.overflow = static_cast<int *>(take(chunk * sizeof(int)));
+  w.phase = static_cast<int *>(take(chunk * sizeof(int)));
+  w.chunk = static_cast<int>(chunk);
+  w.marks = nullptr;
+  return off <= bytes ? 0 : off;
 }
 
 // Anchor / target plan of one model's periods (reference continuation.hpp:12-20,
@@ -282,7 +417,9 @@ int run_continuation(md_rocm_solver *s, int kmax, int max_rows, int wave, double
   MD_HIP(hipMemcpyAsync(s->c_count.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
   MD_HIP(mdr::launch_gather(s->traces.as<mdr::TraceInfo>(), s->c_anchors.as<int>(), na,
                             s->c_list.as<mdr::TraceInfo>(), st));
-  if (int e = run_pipeline(s, na, kmax, max_rows, wave, tol, v64, nullptr, flags, levels, counts, group, gflags,
+  Work wa;
+  if (int e = solver_work(s, na, kmax, wa)) return e;
+  if (int e = run_pipeline(s, wa, na, kmax, max_rows, wave, tol, v64, nullptr, flags, levels, counts, group, gflags,
                            s->c_list.as<mdr::TraceInfo>(), s->c_roots.as<double>()))
     return e;
   mdr::ContArgs c{};
@@ -321,7 +458,9 @@ int run_continuation(md_rocm_solver *s, int kmax, int max_rows, int wave, double
   if (nfb > 0) {
     MD_HIP(mdr::launch_gather(s->traces.as<mdr::TraceInfo>(), s->c_fallback.as<int>(), nfb,
                               s->c_list.as<mdr::TraceInfo>(), st));
-    if (int e = run_pipeline(s, nfb, kmax, max_rows, wave, tol, v64, nullptr, flags, levels, counts, group, gflags,
+    Work wf;
+    if (int e = solver_work(s, nfb, kmax, wf)) return e;
+    if (int e = run_pipeline(s, wf, nfb, kmax, max_rows, wave, tol, v64, nullptr, flags, levels, counts, group, gflags,
                              s->c_list.as<mdr::TraceInfo>(), nullptr))
       return e;
   }
@@ -460,6 +599,7 @@ md_rocm_solver *md_rocm_create(int capacity, int L, int nf, int nmodes, const md
     fail(1, "no HIP device");
     return nullptr;
   }
+  scratch_setup(s->device);
   hipDeviceProp_t p;
   if (hipGetDeviceProperties(&p, s->device) == hipSuccess) s->cus = p.multiProcessorCount;
   if (hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess ||
@@ -479,12 +619,15 @@ int md_rocm_synchronize(md_rocm_solver *s) {
   MD_HIP(hipStreamSynchronize(s->stream));
   float ms = 0.f;
   for (double &x : s->last_ms) x = 0.;
-  if (hipEventElapsedTime(&ms, s->ev_begin, s->ev_end) == hipSuccess) s->last_ms[0] = ms;
+  // Only events the last run recorded are read: a query of an event never
+  // recorded fails and would leave that error as the thread's last HIP error.
+  if (s->timed && hipEventElapsedTime(&ms, s->ev_begin, s->ev_end) == hipSuccess) s->last_ms[0] = ms;
   for (int c = 0; c < s->marked_chunks; ++c) {
     hipEvent_t *m = s->marks.data() + c * mdr::kStageMarks;
     for (int k = 0; k + 1 < mdr::kStageMarks; ++k)
       if (hipEventElapsedTime(&ms, m[k], m[k + 1]) == hipSuccess) s->last_ms[1 + k] += ms;
   }
+  (void)hipGetLastError();
   return 0;
 }
 
@@ -565,7 +708,9 @@ int device_sequence(md_rocm_solver *s, const double *thickness, const double *vp
                           s->stream));
   s->prep_models = B;
   s->prep_rows = L;
-  return run_pipeline(s, ntraces, s->nmodes + 2, s->L, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
+  Work w;
+  if (int e = solver_work(s, ntraces, s->nmodes + 2, w)) return e;
+  return run_pipeline(s, w, ntraces, s->nmodes + 2, s->L, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
                       s->cfg.group_velocity ? group : nullptr, group_flags);
 }
 }  // namespace
@@ -598,12 +743,14 @@ int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double 
       reinterpret_cast<uintptr_t>(counts), reinterpret_cast<uintptr_t>(group), reinterpret_cast<uintptr_t>(group_flags),
       static_cast<uintptr_t>(s->cfg.scan_waves), static_cast<uintptr_t>(s->cfg.consumer_blocks), accept_bits(s)};
   if (s->cfg.graph && s->graph_exec && key == s->graph_key) {
+    s->timed = true;
     MD_HIP(hipEventRecord(s->ev_begin, s->stream));
     MD_HIP(hipGraphLaunch(s->graph_exec, s->stream));
     MD_HIP(hipEventRecord(s->ev_end, s->stream));
     s->marked_chunks = 0;
     return 0;
   }
+  s->timed = true;
   MD_HIP(hipEventRecord(s->ev_begin, s->stream));
   if (int e = device_sequence(s, thickness, vp, vs, rho, B, periods, row_periods, bounds, c_tol_rel, wave, velocity,
                               flags, levels, counts, group, group_flags))
@@ -633,6 +780,34 @@ int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double 
     s->graph_key = key;
   }
   return 0;
+}
+
+int md_rocm_run_device_static(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                              const double *rho, int B, const double *periods, int row_periods,
+                              const double *bounds, double c_tol_rel, int wave, double *velocity, uint32_t *flags,
+                              int32_t *levels, int32_t *counts, double *group, uint32_t *group_flags) {
+  if (!s) return fail(2, "null solver");
+  if (B < 1 || B > s->capacity) return fail(2, "B must be 1..capacity");
+  if (wave != 0 && wave != 1) return fail(2, "wave must be 0 (Rayleigh) or 1 (Love)");
+  if (!(c_tol_rel > 0.) || !std::isfinite(c_tol_rel)) return fail(2, "c_tol_rel must be positive");
+  if (!thickness || !vp || !vs || !rho || !periods || !bounds || !velocity || !flags || !levels || !counts)
+    return fail(2, "null input or output pointer");
+  if (s->cfg.group_velocity && (!group || !group_flags)) return fail(2, "group outputs required");
+  auto *base = static_cast<unsigned char *>(scratch_address(s->device));
+  if (!base) return fail(1, "the static workspace's device address is unavailable on this device");
+  const int L = s->L, T = s->nf, kmax = s->nmodes + 2;
+  Work w;
+  if (const size_t need = carve_work(s, base, kScratchBytes, B, L, T, kmax, w))
+    return fail(2, "the batch needs " + std::to_string(need) + " bytes of static workspace; the limit is " +
+                       std::to_string(kScratchBytes) + " (MD_ROCM_STATIC_SCRATCH_BYTES): split the batch");
+  const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
+  MD_HIP(mdr::launch_prep(thickness, vp, vs, rho, bounds, periods, row_periods, B, L, T, s->nmodes, pp, w.rows,
+                          w.models, w.traces, s->stream));
+  s->prep_models = B;
+  s->prep_rows = L;
+  s->timed = false;
+  return run_pipeline(s, w, static_cast<long long>(B) * T, kmax, L, wave, c_tol_rel, velocity, nullptr, flags,
+                      levels, counts, s->cfg.group_velocity ? group : nullptr, group_flags);
 }
 
 int md_rocm_autotune(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
@@ -721,6 +896,7 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
     if (int e = s->models.ensure(static_cast<size_t>(B) * sizeof(mdr::ModelInfo))) return e;
     if (int e = s->traces.ensure(ntr * sizeof(mdr::TraceInfo))) return e;
     if (int e = s->c_pred.ensure(ntr * sizeof(uint32_t))) return e;
+    s->timed = true;
     MD_HIP(hipEventRecord(s->ev_begin, st));
     const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
     MD_HIP(mdr::launch_prep(s->in_h.as<double>(), s->in_vp.as<double>(), s->in_vs.as<double>(), s->in_rho.as<double>(),
@@ -832,6 +1008,7 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
     if (int e = s->out_gflags.ensure(ncell * sizeof(uint32_t))) return e;
   }
   hipStream_t st = s->stream;
+  s->timed = true;
   MD_HIP(hipEventRecord(s->ev_begin, st));
   MD_HIP(hipMemcpyAsync(s->rows.p, rows.data(), rows.size() * sizeof(mdr::PrepRow), hipMemcpyHostToDevice, st));
   MD_HIP(hipMemcpyAsync(s->models.p, models.data(), models.size() * sizeof(mdr::ModelInfo), hipMemcpyHostToDevice,
@@ -859,8 +1036,8 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
                                  want_group ? s->out_gflags.as<uint32_t>() : nullptr, s->c_pred.as<uint32_t>(),
                                  anchors, targets))
       return e;
-  } else if (int e = run_pipeline(s, ntr, kmax, max_rows, wave, c_tol_rel, s->out_v.as<double>(), nullptr,
-                                  s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(),
+  } else if (int e = solver_pipeline(s, ntr, kmax, max_rows, wave, c_tol_rel, s->out_v.as<double>(),
+                                     s->out_flags.as<uint32_t>(), s->out_levels.as<int32_t>(),
                                   s->out_counts.as<int32_t>(), want_group ? s->out_group.as<double>() : nullptr,
                                   want_group ? s->out_gflags.as<uint32_t>() : nullptr)) {
     return e;
