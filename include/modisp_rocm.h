@@ -280,6 +280,20 @@ int md_rocm_result_masks(const void *velocity, int velocity_is_f32, const uint32
                          double close_root_rtol, uint8_t *status, uint8_t *mode_mask, uint8_t *close_root_mask,
                          uint8_t *close_root_risk, uint8_t *group_mask);
 
+/* Group velocities of one confirmed trace for the continuity review.
+ * Replaces the plugin function _review_groups (reference
+ * cpp/common/review_groups.hpp:39-63, cpp/cuda/review_groups.cu:6-35):
+ * packed = one model's h, vp, vs, rho ([layers] each) followed by its ordered
+ * roots (guards included, at most 256), so size = 4 layers + roots. Writes
+ * group[modes] (NaN where flagged) and flags[modes] with the group-velocity
+ * flag contract (modes beyond the roots: MISSING_ROOT; neighbour midpoints as
+ * windows, lower / upper at the ends) and the number of derivative
+ * evaluations. secular must be 1. Returns 2 on invalid geometry
+ * ("invalid review trace", as the reference), 3 on allocation failure. */
+int md_rocm_review_groups(const double *packed, size_t size, int layers, int modes, int wave, int secular,
+                          double period, double lower, double upper, double group_rtol, double close_rtol,
+                          double *group, uint32_t *flags, int *evaluations);
+
 /* Device helpers (reference cpp/cuda/qe.cu:1140-1142, bindings.cpp:354-357). */
 int md_rocm_available(int device); /* 1 if the device exists and is gfx950-capable */
 int md_rocm_set_device(int device);

@@ -168,6 +168,25 @@ def result_masks(velocity, flags, counts, close_root_rtol, group=None, group_fla
     return status, mode_mask, close, risk, gm
 
 
+def _review_groups(packed, layers, modes, wave, secular, period, lo, hi, rtol, close_rtol):
+    """Group velocities [modes], their flags and the evaluation count of one
+    confirmed trace (packed: the model's h, vp, vs, rho then its ordered
+    roots), on the device. Reference plugin function _review_groups
+    (cpp/common/review_groups.hpp:39-63)."""
+    p = np.asarray(packed)
+    if p.ndim != 1:
+        raise ValueError("invalid review trace")
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    group = np.empty(max(1, int(modes)))
+    flags = np.empty(max(1, int(modes)), dtype=np.uint32)
+    ev = ctypes.c_int(0)
+    _native.check(_native.lib().md_rocm_review_groups(
+        _native.ptr(p), p.size, int(layers), int(modes), int(wave), int(secular), float(period), float(lo),
+        float(hi), float(rtol), float(close_rtol), _native.ptr(group), _native.ptr(flags, _native._u32p),
+        ctypes.byref(ev)))
+    return group[:modes], flags[:modes], int(ev.value)
+
+
 def _rows(a, name):
     a = np.ascontiguousarray(a, dtype=np.float64)
     if a.ndim != 2:

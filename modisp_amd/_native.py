@@ -51,7 +51,7 @@ EXPORTS = ("md_rocm_default_config", "md_rocm_create", "md_rocm_free", "md_rocm_
            "md_rocm_last_error", "md_rocm_probe_grid", "md_rocm_last_stats", "md_rocm_adjoint",
            "md_rocm_set_stream", "md_rocm_autotune", "md_rocm_qe_kernel_report",
            "md_rocm_love_phase_kernel_report", "md_rocm_continuity_screen", "md_rocm_result_masks",
-           "md_rocm_run_device_static")
+           "md_rocm_run_device_static", "md_rocm_review_groups")
 
 
 def lib():
@@ -88,6 +88,10 @@ def lib():
     L.md_rocm_run_device.restype = ctypes.c_int
     L.md_rocm_run_device_static.argtypes = L.md_rocm_run_device.argtypes
     L.md_rocm_run_device_static.restype = ctypes.c_int
+    L.md_rocm_review_groups.argtypes = [_dp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, _dp, _u32p, ctypes.POINTER(ctypes.c_int)]
+    L.md_rocm_review_groups.restype = ctypes.c_int
     L.md_rocm_run_ragged.argtypes = [_vp, _dp, _dp, _dp, _dp, _i64p, ctypes.c_int, _dp, _i64p, _i32p, _dp,
                                      ctypes.c_double, ctypes.c_int, _dp, _u32p, _i32p, _i32p, _dp, _u32p, _u32p]
     L.md_rocm_run_ragged.restype = ctypes.c_int

@@ -1,28 +1,29 @@
 """Front end of the ROCm backend: ``Solver``, ``dispersion_batch``,
-``dispersion`` and the ``DispersionBatch`` result, with the reference's
-argument names, validation messages, result fields and flag values
-(python/modisp/__init__.py:15-139 result types, 142-157 model validation,
-160-271 Solver options and backend selection, 277-507 the solve,
-572-597 the convenience functions).
+``dispersion`` and the ``DispersionBatch`` result.
 
-Only the ``rocm`` backend lives here: ``backend="auto"`` resolves to it when
-a gfx950 device is available and raises ``BackendUnavailableError``
-otherwise -- this package has no CPU path. In the reference the same
-selection falls through to its CPU backend.
+The public surface is the reference's (python/modisp/__init__.py): argument
+names and defaults, validation messages, result fields, status codes and
+flag values. Those are held here as tables; the machinery around them is
+this package's own:
 
-The engine is this package's ``QeSolver`` (FP32 density-adaptive grid scan,
-FP64 recheck, FP64 bracket refinement, analytic FP64 group velocity) for every
-``accuracy`` setting; ``accuracy`` selects the default refinement tolerance
-as in the reference. ``frequency_strategy="anchor_continuation"`` runs the
-engine's anchor continuation under the reference's conditions (not at
-``accuracy="high"``, at least 8 unique periods, an interior target within
-the anchor log-gap limit; python/modisp/__init__.py:324-341) on the sorted
-unique periods, and reports prediction flags.
+* a declarative check list validates models and options;
+* a period plan decides once which periods the device solves, in which
+  order, and how results return to the caller's order;
+* one device engine (``QeSolver``) is kept per geometry and reused for
+  smaller calls; batches larger than the memory budget run in chunks;
+* result masks come from one native pass (``md_rocm_result_masks``);
+* the continuity review of approximate presets is array-oriented: all
+  suspects of a pass go to one FP64 verifier call with one period per row,
+  and acceptance, change detection and write-back are vector operations.
+
+Only the ``rocm`` backend lives here; ``backend="auto"`` resolves to it when a
+gfx950 device is present and raises ``BackendUnavailableError`` otherwise.
 """
 from __future__ import annotations
 
 import threading
 import time
+from dataclasses import dataclass
 from enum import IntEnum, IntFlag
 
 import numpy as np
@@ -30,6 +31,7 @@ import numpy as np
 from . import QeSolver, BackendUnavailableError, available, bounds as _auto_bounds, set_device, STAT_NAMES
 from . import continuity_screen, result_masks
 
+# --- interface tables (values fixed by the reference's API) -------------------------------
 TRACE_UNRESOLVED_BRACKET = 1
 TRACE_RELOCATED = 2
 TRACE_SEARCH_UNCONVERGED = 4
@@ -49,13 +51,9 @@ TRACE_FP64_REPAIR = 32768
 TRACE_CONTINUATION_APPROXIMATE = 65536
 TRACE_AUTO_RELAXED = 1 << 17
 DEFAULT_ACCURACY = "auto"
-_APPROXIMATE_ACCURACIES = ("auto", "balanced", "fast")
-_REFINEMENT = {"legacy": 0, "brent": 1, "chandrupatla": 2}   # QeSolver codes (reference qe.cu:983)
-_C_TOL = {"auto": 1e-5, "high": 1e-6, "balanced": 1e-5, "fast": 1e-4}
 
 
 class ContinuityFlag(IntFlag):
-    """Continuity review bits (reference python/modisp/_continuity.py:6-14)."""
     REVIEWED = 1
     REPAIRED = 2
     UNRESOLVED = 4
@@ -98,50 +96,189 @@ class PredictionFlag(IntFlag):
     REVIEW = 4096
 
 
+@dataclass(frozen=True)
+class _Preset:
+    """One accuracy preset: its engine code, default refinement tolerance and
+    shape sampling tolerance, and whether its searches are approximate."""
+    code: int
+    c_tol: float
+    sampling: float
+    approximate: bool
+
+
+_PRESETS = {"auto": _Preset(4, 1e-5, 1e-2, True), "high": _Preset(0, 1e-6, 1e-3, False),
+            "balanced": _Preset(1, 1e-5, 1e-2, True), "fast": _Preset(2, 1e-4, 5e-2, True),
+            "uniform": _Preset(0, 1e-6, 1e-3, False)}
+_REFINEMENT_CODE = {"legacy": 0, "brent": 1, "chandrupatla": 2}
+_STRATEGIES = ("shape_adaptive", "qe_adaptive", "uniform")
+_CONTINUATION_COUNTERS = (("root_count_unsafe", PredictionFlag.ROOT_COUNT_UNSAFE),
+                          ("anchor_unsafe", PredictionFlag.ANCHOR_UNSAFE),
+                          ("window_unsafe", PredictionFlag.WINDOW_UNSAFE),
+                          ("bracket_failed", PredictionFlag.BRACKET_FAILED),
+                          ("correction_failed", PredictionFlag.CORRECTION_FAILED),
+                          ("audit_failed", PredictionFlag.AUDIT_FAILED))
+# Device bytes per (model, period) of the engine's trace work buffers, and per
+# model row; used to turn workspace_bytes into models per chunk.
+_TRACE_BYTES, _ROW_BYTES, _FIXED_BYTES = 3600, 64, 1 << 20
+
+
+# --- validation --------------------------------------------------------------------------
+def _real_scalar(v):
+    return (np.isscalar(v) and not isinstance(v, (str, bytes, bool, np.bool_)) and not np.iscomplexobj(v)
+            and np.isfinite(v))
+
+
+def _integer(v):
+    return isinstance(v, (int, np.integer)) and not isinstance(v, (bool, np.bool_))
+
+
+def _flag(v):
+    return isinstance(v, (bool, np.bool_))
+
+
+def _check(rules):
+    """Raise ValueError with the message of the first rule whose test fails."""
+    for ok, message in rules:
+        if not ok():
+            raise ValueError(message)
+
+
+def _models(thickness, vp, vs, rho):
+    """[B, L] float64 model arrays, validated by the reference's rules and messages."""
+    named = (("thickness", thickness), ("vp", vp), ("vs", vs), ("rho", rho))
+    arrays = []
+    for name, value in named:
+        _check([(lambda: not np.iscomplexobj(value), f"{name} must be real")])
+        a = np.ascontiguousarray(value, dtype=np.float64)
+        a = a[None, :] if a.ndim == 1 else a
+        _check([(lambda: a.ndim == 2 and a.size > 0 and bool(np.isfinite(a).all()),
+                 f"{name} must be a finite nonempty [B,L] or [L] array")])
+        arrays.append(a)
+    h, a, b, r = arrays
+    solid = b > 0
+    _check([
+        (lambda: all(x.shape == h.shape for x in arrays), "model array shapes must agree"),
+        (lambda: h.shape[1] <= 256, "at most 256 layers are supported"),
+        (lambda: not (np.any(h < 0) or np.any(a <= 0) or np.any(r <= 0) or np.any(b < 0)),
+         "thickness/vs must be non-negative and vp/rho positive"),
+        (lambda: not (np.any(b[:, -1] <= 0) or np.any(b[:, 1:] == 0)),
+         "only a single first water layer may have vs=0; the half-space must be solid"),
+        (lambda: not np.any(a[solid] / b[solid] <= np.sqrt(4. / 3.)),
+         "solid vp/vs must exceed sqrt(4/3) for positive bulk modulus"),
+    ])
+    return arrays
+
+
+def _periods(periods):
+    _check([(lambda: not np.iscomplexobj(periods), "periods must be real")])
+    per = np.ascontiguousarray(periods, dtype=np.float64)
+    _check([(lambda: per.ndim == 1 and per.size > 0 and bool(np.isfinite(per).all()) and not np.any(per <= 0),
+             "periods must be a finite positive 1-D array")])
+    return per
+
+
+def _fp32_range(arrays, per):
+    f = np.finfo(np.float32)
+    big, tiny = float(f.max), .5 * float(f.smallest_subnormal)
+    for name, x in zip(("thickness", "vp", "vs", "rho"), arrays):
+        _check([(lambda: not (x.max() > big or np.any((x > 0.) & (x <= tiny))),
+                 f"{name} contains values outside the float32 range")])
+    _check([(lambda: not (per.max() > big or per.min() < 2. * np.pi / big),
+             "periods or their angular frequencies are outside the float32 range")])
+
+
+# --- period plan -------------------------------------------------------------------------
+@dataclass
+class _PeriodPlan:
+    """Which periods the device solves (`grid`), whether by anchor
+    continuation, and the index that restores the caller's order."""
+    grid: np.ndarray
+    restore: np.ndarray | None
+    continuation: bool
+    reason: str
+
+    @classmethod
+    def build(cls, per, solver, backend):
+        rising = per.size < 2 or bool(np.all(np.diff(per) > 0))
+        falling = per.size >= 2 and bool(np.all(np.diff(per) < 0))
+        wants = solver.frequency_strategy == "anchor_continuation"
+        # Continuation needs its periods sorted; any other repeated or
+        # unsorted (but not strictly falling) grid is solved once per unique
+        # period. A strictly falling grid keeps the caller's order.
+        needs_sorted = wants and solver.accuracy != "high" and per.size >= 8
+        unique, inverse = per, None
+        if not rising and (needs_sorted or not falling):
+            unique, inverse = np.unique(per, return_inverse=True)
+        continuation, reason = False, "independent frequency search requested"
+        if wants:
+            near = unique.size >= 3 and bool(np.any(np.log(unique[2:]) - np.log(unique[:-2]) <= .25))
+            for blocked, why in ((solver.accuracy == "high", "high accuracy retains independent search"),
+                                 (unique.size < 8, "fewer than 8 unique periods; using independent search"),
+                                 (not near, "period grid has no interior target within the anchor log-gap limit; "
+                                            "using independent search")):
+                if blocked:
+                    reason = why
+                    break
+            else:
+                continuation, reason = True, f"{backend.upper()} anchor continuation on sorted unique periods"
+        if continuation or unique.size < per.size:
+            return cls(unique, inverse, continuation, reason)
+        return cls(per, None, continuation, reason)
+
+    def back(self, x):
+        return x if self.restore is None or x is None else x.take(self.restore, axis=-1)
+
+
+# --- results -------------------------------------------------------------------------------
 class DispersionBatch:
     """Phase (and group) velocities [B, nmodes, T] with the reference's masks,
-    status codes and per-trace diagnostics."""
+    status codes and per-trace diagnostics (field names as in
+    python/modisp/__init__.py:71-139)."""
 
-    def __init__(self, velocity, periods, nmodes, wave, flags, levels, counts, bounds, backend, reason,
-                 accuracy=None, close_root_rtol=.005, group_velocity=None, group_velocity_flags=None,
-                 prediction_flags=None, group_rtol=None, frequency_strategy="independent",
-                 frequency_strategy_used="independent",
-                 frequency_strategy_reason="independent frequency search requested", anchor_stride=4,
-                 continuity_flags=None, continuity_unresolved=False):
+    def __init__(self, velocity, periods, nmodes, wave, flags, levels, counts, bounds, backend, reason, **meta):
         self.phase_velocity = velocity
         self.precision = velocity.dtype.name
-        self.period = periods.copy()
+        self.period = np.array(periods, copy=True)
         self.mode = np.arange(nmodes)
         self.wave = wave
-        self.accuracy = accuracy
-        self.close_root_rtol = float(close_root_rtol)
-        self.group_velocity = group_velocity
-        self.group_velocity_flags = group_velocity_flags
-        (self.status, self.mode_mask, self.close_root_mask, self.close_root_risk,
-         self.group_velocity_mask) = result_masks(velocity, flags, counts, close_root_rtol, group_velocity,
-                                                  group_velocity_flags)
-        self.group_velocity_valid = self.group_velocity_mask
-        self.prediction_flags = prediction_flags
-        self.continuity_flags = continuity_flags
-        self.group_rtol = group_rtol
-        self.frequency_strategy = frequency_strategy
-        self.frequency_strategy_used = frequency_strategy_used
-        self.frequency_strategy_reason = frequency_strategy_reason
-        self.anchor_stride = int(anchor_stride)
-        self.approximate_search = (flags & (TRACE_APPROXIMATE_SEARCH | TRACE_CONTINUATION_APPROXIMATE)) != 0
-        self.mode_index_uncertain = self.approximate_search.copy()
-        if continuity_unresolved:
-            self.mode_index_uncertain |= (continuity_flags & ContinuityFlag.UNRESOLVED) != 0
-        self.potentially_ambiguous = self.mode_index_uncertain | self.close_root_risk
         self.trace_flags = flags
-        self.numerical_flags = flags & np.uint32(0xFFFFFFFF ^ (TRACE_RANGE_TRUNCATED | TRACE_AUTO_RELAXED))
         self.search_level = levels
         self.root_count = counts
-        self.search_bounds = bounds.copy()
+        self.search_bounds = np.array(bounds, copy=True)
         self.backend_used = backend
         self.backend_reason = reason
+        defaults = dict(accuracy=None, close_root_rtol=.005, search_strategy=None, sampling_rtol=None, secular=None,
+                        refinement=None, love_strategy=None, group_velocity=None, group_velocity_flags=None,
+                        prediction_flags=None, group_rtol=None, frequency_strategy="independent",
+                        frequency_strategy_used="independent",
+                        frequency_strategy_reason="independent frequency search requested", anchor_stride=4,
+                        continuity_flags=None, continuity_unresolved=False)
+        unknown = set(meta) - set(defaults)
+        if unknown:
+            raise TypeError(f"unknown result fields {sorted(unknown)}")
+        defaults.update(meta)
+        unresolved = defaults.pop("continuity_unresolved")
+        for k, v in defaults.items():
+            setattr(self, k, v)
+        self.close_root_rtol = float(self.close_root_rtol)
+        self.anchor_stride = int(self.anchor_stride)
+        (self.status, self.mode_mask, self.close_root_mask, self.close_root_risk,
+         self.group_velocity_mask) = result_masks(velocity, flags, counts, self.close_root_rtol, self.group_velocity,
+                                                  self.group_velocity_flags)
+        self.group_velocity_valid = self.group_velocity_mask
+        self.numerical_flags = flags & np.uint32(0xFFFFFFFF ^ (TRACE_RANGE_TRUNCATED | TRACE_AUTO_RELAXED))
+        self.approximate_search = (flags & (TRACE_APPROXIMATE_SEARCH | TRACE_CONTINUATION_APPROXIMATE)) != 0
+        uncertain = self.approximate_search.copy()
+        if unresolved and self.continuity_flags is not None:
+            uncertain |= (self.continuity_flags & ContinuityFlag.UNRESOLVED) != 0
+        self.mode_index_uncertain = uncertain
+        self.potentially_ambiguous = uncertain | self.close_root_risk
         self.mode_convention = ("ascending recovered roots; close-cluster omissions may shift indices"
-                                if np.any(self.approximate_search) else "ascending roots in the reported search interval")
+                                if self.approximate_search.any() else "ascending roots in the reported search interval")
+
+    @property
+    def auto_relaxed(self):
+        return (self.trace_flags & TRACE_AUTO_RELAXED) != 0 if self.accuracy == "auto" else None
 
     @property
     def n_models(self):
@@ -159,43 +296,37 @@ class DispersionBatch:
                 f"nmodes={self.n_modes} T={self.period.size}>")
 
 
-def _models(thickness, vp, vs, rho):
-    """Reference model validation (python/modisp/__init__.py:142-157)."""
-    arrays = []
-    for name, value in zip(("thickness", "vp", "vs", "rho"), (thickness, vp, vs, rho)):
-        if np.iscomplexobj(value):
-            raise ValueError(f"{name} must be real")
-        a = np.ascontiguousarray(value, dtype=np.float64)
-        if a.ndim == 1:
-            a = a[None, :]
-        if a.ndim != 2 or not a.size or not np.isfinite(a).all():
-            raise ValueError(f"{name} must be a finite nonempty [B,L] or [L] array")
-        arrays.append(a)
-    h, a, b, r = arrays
-    if any(v.shape != h.shape for v in arrays):
-        raise ValueError("model array shapes must agree")
-    if h.shape[1] > 256:
-        raise ValueError("at most 256 layers are supported")
-    if np.any(h < 0) or np.any(a <= 0) or np.any(r <= 0) or np.any(b < 0):
-        raise ValueError("thickness/vs must be non-negative and vp/rho positive")
-    if np.any(b[:, -1] <= 0) or np.any(b[:, 1:] == 0):
-        raise ValueError("only a single first water layer may have vs=0; the half-space must be solid")
-    solid = b > 0
-    if np.any(a[solid] / b[solid] <= np.sqrt(4. / 3.)):
-        raise ValueError("solid vp/vs must exceed sqrt(4/3) for positive bulk modulus")
-    return arrays
+class _Outputs:
+    """Result arrays of one solve, filled chunk by chunk (or taken whole from a
+    single chunk's engine output)."""
+
+    def __init__(self):
+        self.velocity = self.flags = self.levels = self.counts = None
+        self.group = self.group_flags = self.prediction = None
+
+    def put(self, sl, B, out, with_group, with_prediction):
+        velocity, flags, _, levels, counts, extras = out
+        parts = dict(velocity=velocity, flags=flags, levels=levels, counts=counts,
+                     group=extras["group_velocity"] if with_group else None,
+                     group_flags=extras["group_velocity_flags"] if with_group else None,
+                     prediction=extras["prediction_flags"] if with_prediction else None)
+        whole = sl.start == 0 and sl.stop == B
+        for name, part in parts.items():
+            if part is None:
+                continue
+            if whole:
+                setattr(self, name, part)
+                continue
+            if getattr(self, name) is None:
+                setattr(self, name, np.empty((B,) + part.shape[1:], dtype=part.dtype))
+            getattr(self, name)[sl] = part
 
 
+# --- solver --------------------------------------------------------------------------------
 class Solver:
-    """Reusable phase dispersion solver on the ROCm backend; use one instance
-    per concurrent caller (python/modisp/__init__.py:160 for the option set).
-
-    ``refinement`` selects the bracket refinement (Brent-Dekker, Chandrupatla or
-    the legacy regula falsi, as in the reference). The options the reference
-    uses to pick among its CPU/CUDA search strategies (search_strategy,
-    sampling_rtol, secular, love_strategy, adaptive_period_threshold, threads,
-    block, nc) are validated and recorded; this backend has one device search
-    for all of them."""
+    """Reusable phase dispersion solver on the ROCm backend; one instance per
+    concurrent caller. Options are the reference's (python/modisp/__init__.py:
+    160-271); see `_engine_options` for what each one selects here."""
 
     def __init__(self, nc=512, cmin=None, cmax=None, c_tol_rel=None, f_accept=1., block=256,
                  search_strategy="shape_adaptive", workspace_bytes=256 * 1024 ** 2, chunk_size=None, *,
@@ -203,65 +334,73 @@ class Solver:
                  adaptive_period_threshold=None, close_root_rtol=.005, sampling_rtol=None, secular=None,
                  refinement=None, love_strategy="phase", precision="float64", group_velocity=False,
                  group_rtol=.001, frequency_strategy="independent", anchor_stride=4, continuity_check=True):
-        if backend not in ("auto", "cpu", "cuda", "rocm"):
-            raise ValueError("backend must be auto, cpu, cuda or rocm")
-        if search_strategy not in ("shape_adaptive", "qe_adaptive", "uniform"):
-            raise ValueError("invalid search_strategy")
-        if not isinstance(continuity_check, (bool, np.bool_)):
-            raise ValueError("continuity_check must be boolean")
-        if not isinstance(group_velocity, (bool, np.bool_)):
-            raise ValueError("group_velocity must be boolean")
-        if (not np.isscalar(group_rtol) or isinstance(group_rtol, (str, bytes, bool, np.bool_))
-                or np.iscomplexobj(group_rtol) or not np.isfinite(group_rtol) or not 0 < group_rtol < 1):
-            raise ValueError("group_rtol must be finite and between 0 and 1")
-        if frequency_strategy not in ("independent", "anchor_continuation"):
-            raise ValueError("frequency_strategy must be independent or anchor_continuation")
-        if (not isinstance(anchor_stride, (int, np.integer)) or isinstance(anchor_stride, (bool, np.bool_))
-                or anchor_stride < 2):
-            raise ValueError("anchor_stride must be an integer of at least 2")
-        if not isinstance(precision, str) or precision not in ("float64", "float32"):
-            raise ValueError("precision must be float64 or float32")
-        if secular not in (None, "dunkin", "fast"):
-            raise ValueError("secular must be None, 'dunkin' or 'fast'")
-        if refinement is None:
-            refinement = "brent"
-        if refinement not in ("legacy", "brent", "chandrupatla"):
-            raise ValueError("refinement must be legacy, brent or chandrupatla")
-        if love_strategy not in ("phase", "scan"):
-            raise ValueError("love_strategy must be phase or scan")
-        if accuracy is None:
-            accuracy = DEFAULT_ACCURACY
-        elif accuracy not in ("auto", "high", "balanced", "fast"):
-            raise ValueError("accuracy must be auto, high, balanced or fast")
-        if not np.isscalar(close_root_rtol) or np.iscomplexobj(close_root_rtol) or not np.isfinite(close_root_rtol) \
-                or not 0 < close_root_rtol < 1:
-            raise ValueError("close_root_rtol must be finite and between 0 and 1")
-        if c_tol_rel is None:
-            c_tol_rel = _C_TOL[accuracy]
-        for name, v in (("cmin", cmin), ("cmax", cmax)):
-            if v is not None and (not np.isfinite(v) or v <= 0):
-                raise ValueError(f"{name} must be finite and positive")
-        if cmin is not None and cmax is not None and cmax <= cmin:
-            raise ValueError("cmax must exceed cmin")
-        if not np.isfinite(c_tol_rel) or c_tol_rel <= 0:
-            raise ValueError("c_tol_rel must be positive")
-        if not np.isfinite(f_accept) or f_accept < 0:
-            raise ValueError("f_accept must be non-negative")
-        if nc < 3:
-            raise ValueError("nc must be at least 3")
-        if not isinstance(workspace_bytes, (int, np.integer)) or workspace_bytes < 1:
-            raise ValueError("workspace_bytes must be a positive integer")
-        if chunk_size is not None and (not isinstance(chunk_size, (int, np.integer)) or chunk_size < 1):
-            raise ValueError("chunk_size must be positive")
-        if not isinstance(device, (int, np.integer)) or device < 0:
-            raise ValueError("device must be a non-negative integer")
-        if not isinstance(profile, (bool, np.bool_)):
-            raise ValueError("profile must be boolean")
-        self.nc, self.cmin, self.cmax = int(nc), cmin, cmax
+        uniform = search_strategy == "uniform"
+        _check([
+            (lambda: backend in ("auto", "cpu", "cuda", "rocm"), "backend must be auto, cpu, cuda or rocm"),
+            (lambda: search_strategy in _STRATEGIES, "invalid search_strategy"),
+            (lambda: _flag(continuity_check), "continuity_check must be boolean"),
+            (lambda: _flag(group_velocity), "group_velocity must be boolean"),
+            (lambda: _real_scalar(group_rtol) and 0 < group_rtol < 1,
+             "group_rtol must be finite and between 0 and 1"),
+            (lambda: frequency_strategy in ("independent", "anchor_continuation"),
+             "frequency_strategy must be independent or anchor_continuation"),
+            (lambda: _integer(anchor_stride) and anchor_stride >= 2, "anchor_stride must be an integer of at least 2"),
+            (lambda: isinstance(precision, str) and precision in ("float64", "float32"),
+             "precision must be float64 or float32"),
+            (lambda: not (precision == "float32" and (backend == "cpu" or uniform)),
+             "precision='float32' requires an adaptive CUDA solver"),
+            (lambda: secular in (None, "dunkin", "fast"), "secular must be None, 'dunkin' or 'fast'"),
+            (lambda: refinement in (None, "legacy", "brent", "chandrupatla"),
+             "refinement must be legacy, brent or chandrupatla"),
+            (lambda: love_strategy in ("phase", "scan"), "love_strategy must be phase or scan"),
+        ])
+        if uniform and (group_velocity or frequency_strategy != "independent"):
+            raise NotImplementedError("group velocity and frequency continuation require an adaptive solver")
+        refinement = refinement or ("legacy" if uniform else "brent")
+        if uniform and refinement != "legacy":
+            raise NotImplementedError("uniform supports only legacy refinement")
+        if uniform and secular == "fast":
+            raise NotImplementedError("secular='fast' is not available on the legacy uniform CUDA path")
+        if accuracy is None or (uniform and accuracy == "auto"):
+            accuracy = "uniform" if uniform else DEFAULT_ACCURACY
+        elif uniform:
+            raise ValueError("accuracy applies only to adaptive strategies; uniform is the legacy fixed path")
+        _check([(lambda: accuracy in ("auto", "high", "balanced", "fast", "uniform"),
+                 "accuracy must be auto, high, balanced or fast")])
+        preset = _PRESETS[accuracy]
+        if adaptive_period_threshold is not None:
+            _check([(lambda: accuracy not in ("high", "uniform"),
+                     "adaptive_period_threshold applies only to auto/balanced/fast"),
+                    (lambda: _real_scalar(adaptive_period_threshold) and adaptive_period_threshold > 0,
+                     "adaptive_period_threshold must be finite and positive")])
+        _check([(lambda: _real_scalar(close_root_rtol) and 0 < close_root_rtol < 1,
+                 "close_root_rtol must be finite and between 0 and 1"),
+                (lambda: sampling_rtol is None or search_strategy == "shape_adaptive",
+                 "sampling_rtol applies only to shape_adaptive")])
+        self._auto_sampling = accuracy == "auto" and search_strategy == "shape_adaptive" and sampling_rtol is None
+        sampling_rtol = preset.sampling if sampling_rtol is None else sampling_rtol
+        c_tol_rel = preset.c_tol if c_tol_rel is None else c_tol_rel
+        _check([
+            (lambda: _real_scalar(sampling_rtol) and 0 < sampling_rtol <= 1,
+             "sampling_rtol must be finite and between 0 and 1"),
+            (lambda: cmin is None or (np.isfinite(cmin) and cmin > 0), "cmin must be finite and positive"),
+            (lambda: cmax is None or (np.isfinite(cmax) and cmax > 0), "cmax must be finite and positive"),
+            (lambda: cmin is None or cmax is None or cmax > cmin, "cmax must exceed cmin"),
+            (lambda: np.isfinite(c_tol_rel) and c_tol_rel > 0, "c_tol_rel must be positive"),
+            (lambda: np.isfinite(f_accept) and f_accept >= 0, "f_accept must be non-negative"),
+            (lambda: nc >= 3, "nc must be at least 3"),
+            (lambda: _integer(workspace_bytes) and workspace_bytes >= 1, "workspace_bytes must be a positive integer"),
+            (lambda: chunk_size is None or (_integer(chunk_size) and chunk_size >= 1), "chunk_size must be positive"),
+            (lambda: _integer(device) and device >= 0, "device must be a non-negative integer"),
+            (lambda: threads is None or (_integer(threads) and 1 <= threads <= 256), "threads must be 1..256"),
+            (lambda: _flag(profile), "profile must be boolean"),
+        ])
+        self.nc, self.cmin, self.cmax, self.block = int(nc), cmin, cmax, int(block)
         self.c_tol_rel, self.f_accept = float(c_tol_rel), float(f_accept)
         self.search_strategy, self.workspace_bytes, self.chunk_size = search_strategy, int(workspace_bytes), chunk_size
         self.backend, self.device, self.threads, self.profile = backend, int(device), threads, bool(profile)
-        self.accuracy, self.close_root_rtol = accuracy, float(close_root_rtol)
+        self.accuracy, self.adaptive_period_threshold = accuracy, adaptive_period_threshold
+        self.close_root_rtol, self.sampling_rtol = float(close_root_rtol), float(sampling_rtol)
         self.secular, self.refinement, self.love_strategy = secular, refinement, love_strategy
         self.precision, self.group_velocity, self.group_rtol = precision, bool(group_velocity), float(group_rtol)
         self.frequency_strategy, self.anchor_stride = frequency_strategy, int(anchor_stride)
@@ -270,11 +409,45 @@ class Solver:
         self._engine_key = None
         self._calls = 0
         self._last_stats = None
-        self.last_result = None
         self._key = None
+        self.last_result = None
         self._lock = threading.RLock()
 
-    # -- backend selection (reference __init__.py:255-271) --------------------------------------
+    # -- engine ----------------------------------------------------------------------------
+    def _engine_options(self, continuation):
+        """QeSolver arguments: the preset's code and strategy (search level
+        codes and approximate flags), the refinement and Love methods, the
+        requested precision for the scan and the returned velocities, and, for
+        search_strategy="uniform", the legacy nc-node uniform grid."""
+        uniform = self.search_strategy == "uniform"
+        threshold = self.adaptive_period_threshold or (1.0 if self.accuracy in ("balanced", "auto") else
+                                                       .63 if self.accuracy == "fast" else 0.)
+        return dict(accuracy=_PRESETS[self.accuracy].code, period_threshold=float(threshold),
+                    close_root_rtol=self.close_root_rtol, shape=self.search_strategy == "shape_adaptive",
+                    sampling_rtol=-1. if self._auto_sampling else self.sampling_rtol,
+                    secular=0 if self.secular == "dunkin" else 1, refinement=_REFINEMENT_CODE[self.refinement],
+                    love_phase=int(self.love_strategy == "phase"), single_precision=self.precision == "float32",
+                    group_velocity=self.group_velocity, group_rtol=self.group_rtol, continuation=continuation,
+                    anchor_stride=self.anchor_stride, grid="uniform" if uniform else "adaptive", nc=self.nc,
+                    scan="float32" if uniform else None)
+
+    def _capacity(self, B, L, T):
+        per_model = T * _TRACE_BYTES + L * _ROW_BYTES
+        fit = (self.workspace_bytes - _FIXED_BYTES) // per_model
+        if fit < 1:
+            raise MemoryError("workspace_bytes is too small for one model and the backend's fixed workspace; "
+                              "increase the budget or reduce threads")
+        return int(min(B, fit, self.chunk_size or B))
+
+    def _acquire(self, capacity, L, T, nmodes, continuation):
+        key = (L, T, int(nmodes), continuation, tuple(sorted(self._engine_options(continuation).items())))
+        if self._engine is not None and self._engine_key[0] == key and self._engine_key[1] >= capacity:
+            return self._engine_key[1], False
+        self.close()
+        self._engine = QeSolver(capacity, L, T, int(nmodes), **self._engine_options(continuation))
+        self._engine_key = (key, capacity)
+        return capacity, True
+
     def _select(self):
         if self.backend in ("cpu", "cuda"):
             raise BackendUnavailableError(f"backend '{self.backend}' is not part of the ROCm plugin; "
@@ -313,177 +486,124 @@ class Solver:
         finally:
             self._lock.release()
 
+    # -- one solve ------------------------------------------------------------------------
     def _solve(self, thickness, vp, vs, rho, periods, nmodes, wave, _bounds=None, _row_periods=None):
-        """One solve. The private options serve the continuity review: explicit
-        search bounds [B, 2] and one period per model (`periods` then holds
-        that list's first entry, as in the reference's verifier call)."""
-        started = time.perf_counter()
-        if wave not in ("rayleigh", "love"):
-            raise ValueError("wave must be rayleigh or love")
-        if not isinstance(nmodes, (int, np.integer)) or not 1 <= nmodes <= 256:
-            raise ValueError("nmodes must be 1..256")
-        h, a, b, r = _models(thickness, vp, vs, rho)
-        B, L = h.shape
-        if np.iscomplexobj(periods):
-            raise ValueError("periods must be real")
-        per = np.ascontiguousarray(periods, dtype=np.float64)
-        if per.ndim != 1 or not per.size or not np.isfinite(per).all() or np.any(per <= 0):
-            raise ValueError("periods must be a finite positive 1-D array")
+        """One solve. `_bounds` ([B, 2]) and `_row_periods` (one period per
+        model, `periods` then holding the first) serve the continuity review."""
+        clock = time.perf_counter()
+        _check([(lambda: wave in ("rayleigh", "love"), "wave must be rayleigh or love"),
+                (lambda: _integer(nmodes) and 1 <= nmodes <= 256, "nmodes must be 1..256")])
+        models = _models(thickness, vp, vs, rho)
+        per = _periods(periods)
+        if self.precision == "float32":
+            _fp32_range(models, per)
+        B, L = models[0].shape
+        uniform = self.search_strategy == "uniform"
+        if uniform and (wave != "rayleigh" or np.any(models[2] == 0) or L < 2):
+            raise NotImplementedError("uniform is the legacy CUDA solid-Rayleigh option; "
+                                      "use qe_adaptive for this combination")
         backend, reason = self._select()
-        input_periods = per
-        unique_periods = per
-        inverse = None
-        # Unsorted or repeated periods are solved once each (reference 314-341, restored at 463-475);
-        # continuation needs them sorted.
-        if per.size > 1 and not np.all(per[1:] > per[:-1]):
-            needs_sorted = self.frequency_strategy == "anchor_continuation" and self.accuracy != "high" \
-                and per.size >= 8
-            if needs_sorted or not np.all(per[1:] < per[:-1]):
-                unique_periods, inverse = np.unique(per, return_inverse=True)
-        continuation = False
-        frequency_reason = "independent frequency search requested"
-        if self.frequency_strategy == "anchor_continuation":
-            if self.accuracy == "high":
-                frequency_reason = "high accuracy retains independent search"
-            elif unique_periods.size < 8:
-                frequency_reason = "fewer than 8 unique periods; using independent search"
-            elif not np.any(np.log(unique_periods[2:]) - np.log(unique_periods[:-2]) <= .25):
-                frequency_reason = ("period grid has no interior target within the anchor log-gap limit; "
-                                    "using independent search")
-            else:
-                continuation = True
-                frequency_reason = "ROCM anchor continuation on sorted unique periods"
-        if continuation or unique_periods.size < per.size:
-            per = unique_periods
-        else:
-            inverse = None  # keep the caller's order for independent, all-unique grids
-        frequency_used = "anchor_continuation" if continuation else "independent"
-        wave_code = int(wave == "love")
-        bnd = (_auto_bounds(h, a, b, r, wave_code, self.cmin, self.cmax) if _bounds is None
-               else np.ascontiguousarray(_bounds, dtype=np.float64))
-        if not np.isfinite(bnd).all() or np.any(bnd[:, 0] <= 0):
-            raise ValueError("automatic search range could not be constructed")
-        row_periods = None
+        plan = _PeriodPlan.build(per, self, backend)
         if _row_periods is not None:
-            row_periods = np.ascontiguousarray(_row_periods, dtype=np.float64).reshape(B, 1)
-            per, inverse, continuation = per[:1], None, False
-            frequency_used, frequency_reason = "independent", "independent frequency search requested"
-        capacity = min(B, self.chunk_size or B)
-        key = (capacity, L, per.size, int(nmodes), self.precision, self.group_velocity, self.group_rtol,
-               self.close_root_rtol, continuation, self.anchor_stride, self.refinement)
-        allocated = self._engine_key is None or self._engine_key[1:] != key[1:] or self._engine_key[0] < capacity
-        if allocated:
-            self.close()
-            self._engine = QeSolver(capacity, L, per.size, int(nmodes), close_root_rtol=self.close_root_rtol,
-                                    single_precision=self.precision == "float32",
-                                    group_velocity=self.group_velocity, group_rtol=self.group_rtol,
-                                    continuation=continuation, anchor_stride=self.anchor_stride,
-                                    refinement=_REFINEMENT[self.refinement])
-            self._engine_key = key
-        else:
-            capacity = self._engine_key[0]
-        # One chunk: the engine's output arrays become the result as they are.
-        single = capacity >= B
-        if not single:
-            velocity = np.empty((B, nmodes, per.size), dtype=self.precision)
-            flags = np.empty((B, per.size), dtype=np.uint32)
-            levels = np.empty((B, per.size), dtype=np.int32)
-            counts = np.empty_like(levels)
-        group = gflags = None
-        if self.group_velocity and not single:
-            group = np.empty(velocity.shape, dtype=np.float64)
-            gflags = np.empty(velocity.shape, dtype=np.uint32)
-        prediction = (np.full((B, per.size), int(PredictionFlag.BYPASSED), dtype=np.uint32)
-                      if self.frequency_strategy == "anchor_continuation" else None)
-        stats = {k: 0 for k in STAT_NAMES}
-        chunks = 0
+            plan = _PeriodPlan(per[:1], None, False, "independent frequency search requested")
+        wave_code = int(wave == "love")
+        low = .2 if uniform and self.cmin is None else self.cmin
+        bnd = (_auto_bounds(*models, wave_code, low, self.cmax) if _bounds is None
+               else np.ascontiguousarray(_bounds, dtype=np.float64))
+        _check([(lambda: bool(np.isfinite(bnd).all()) and not np.any(bnd[:, 0] <= 0),
+                 "automatic search range could not be constructed")])
+        T = plan.grid.size
+        capacity, allocated = self._acquire(self._capacity(B, L, T), L, T, nmodes, plan.continuation)
+        rows = None if _row_periods is None else np.ascontiguousarray(_row_periods, dtype=np.float64).reshape(B, 1)
+        out, stats, chunks = _Outputs(), {k: 0 for k in STAT_NAMES}, 0
         for start in range(0, B, capacity):
-            end = min(B, start + capacity)
-            chunk_periods = per if row_periods is None else row_periods[start:end]
-            out = self._engine.run(h[start:end], a[start:end], b[start:end], r[start:end], chunk_periods,
-                                   bnd[start:end], self.c_tol_rel, self.f_accept, wave_code, self.cmin is None,
-                                   self.profile)
-            if single:
-                velocity, flags, part, levels, counts, extras = out
-                if self.group_velocity:
-                    group, gflags = extras["group_velocity"], extras["group_velocity_flags"]
-            else:
-                velocity[start:end], flags[start:end], part, levels[start:end], counts[start:end], extras = out
-                if self.group_velocity:
-                    group[start:end] = extras["group_velocity"]
-                    gflags[start:end] = extras["group_velocity_flags"]
-            if continuation:
-                chunk_prediction = extras["prediction_flags"]
-                prediction[start:end] = chunk_prediction
-                for name, bit in (("root_count_unsafe", 128), ("anchor_unsafe", 256), ("window_unsafe", 512),
-                                  ("bracket_failed", 1024), ("correction_failed", 2048), ("audit_failed", 32)):
-                    counter = "continuation_fallback_" + name
-                    stats[counter] = stats.get(counter, 0) + int(np.count_nonzero(chunk_prediction & bit))
-                stats["continuation_accepted"] = stats.get("continuation_accepted", 0) + int(
-                    np.count_nonzero(chunk_prediction & int(PredictionFlag.ACCEPTED)))
-                stats["continuation_fallbacks"] = stats.get("continuation_fallbacks", 0) + int(
-                    np.count_nonzero(chunk_prediction & int(PredictionFlag.FALLBACK)))
-            for k, v in part.items():
+            sl = slice(start, min(B, start + capacity))
+            res = self._engine.run(*(m[sl] for m in models), plan.grid if rows is None else rows[sl], bnd[sl],
+                                   self.c_tol_rel, self.f_accept, wave_code, self.cmin is None, self.profile)
+            out.put(sl, B, res, self.group_velocity, plan.continuation)
+            for k, v in res[2].items():
                 stats[k] = stats.get(k, 0) + int(v)
             chunks += 1
-        # Continuity screen and FP64 review of suspect traces (reference 445-460).
-        continuity = None
-        if self.accuracy in _APPROXIMATE_ACCURACIES and (self.continuity_check or continuation
-                                                         or self.accuracy == "auto"):
-            continuity, suspect = continuity_screen(per, velocity, prediction, self.continuity_check)
-            if self.accuracy == "auto":
-                failure = (flags & (TRACE_UNRESOLVED_BRACKET | TRACE_SEARCH_UNCONVERGED | TRACE_ROOT_CONFLICT)) != 0
-                if np.any(failure):
-                    continuity[failure] |= int(ContinuityFlag.SEARCH)
-                    suspect = True
-            stats.update(continuity_suspected=0, continuity_local_searches=0, continuity_full_searches=0,
-                         continuity_repaired=0, continuity_unresolved=0, continuity_evaluations=0)
-            if suspect:
-                _review(self, (h, a, b, r), per, wave, bnd, velocity, flags, levels, counts, group, gflags,
-                        continuity, stats)
-        computed_periods = per.size
-        if inverse is not None:
-            velocity, flags, levels, counts = (x.take(inverse, axis=-1) for x in (velocity, flags, levels, counts))
-            if group is not None:
-                group, gflags = group.take(inverse, axis=-1), gflags.take(inverse, axis=-1)
-            if prediction is not None:
-                prediction = prediction.take(inverse, axis=-1)
-            if continuity is not None:
-                continuity = continuity.take(inverse, axis=-1)
+        prediction = out.prediction
+        if self.frequency_strategy == "anchor_continuation" and prediction is None:
+            prediction = np.full((B, T), int(PredictionFlag.BYPASSED), dtype=np.uint32)
+        if plan.continuation:
+            for name, bit in _CONTINUATION_COUNTERS:
+                stats["continuation_fallback_" + name] = int(np.count_nonzero(prediction & int(bit)))
+            stats["continuation_accepted"] = int(np.count_nonzero(prediction & int(PredictionFlag.ACCEPTED)))
+            stats["continuation_fallbacks"] = int(np.count_nonzero(prediction & int(PredictionFlag.FALLBACK)))
+        continuity = self._continuity(models, plan, wave, bnd, out, prediction, stats)
+        velocity, flags, levels, counts = (plan.back(x) for x in (out.velocity, out.flags, out.levels, out.counts))
+        group, gflags, prediction, continuity = (plan.back(x) for x in (out.group, out.group_flags, prediction,
+                                                                        continuity))
+        if uniform:
+            flags |= np.uint32(TRACE_RANGE_TRUNCATED)
+            levels[:] = -2
         if self.cmin is not None or self.cmax is not None:
-            flags |= TRACE_RANGE_TRUNCATED
+            flags |= np.uint32(TRACE_RANGE_TRUNCATED)
         counts[(flags & TRACE_SEARCH_UNCONVERGED) != 0] = -1
-        stats.update(backend=backend, precision=self.precision, accuracy=self.accuracy,
-                     strategy="rocm_adaptive_scan", requested_strategy=self.search_strategy,
-                     refinement=self.refinement, love_strategy=self.love_strategy if wave == "love" else None,
-                     close_root_rtol=self.close_root_rtol, refinement_tolerance=self.c_tol_rel, chunks=chunks,
-                     capacity_models=int(capacity), engine_allocations=int(allocated),
-                     group_velocity=self.group_velocity, group_rtol=self.group_rtol,
-                     frequency_strategy=self.frequency_strategy, frequency_strategy_used=frequency_used,
-                     frequency_strategy_reason=frequency_reason, anchor_stride=self.anchor_stride,
-                     periods_requested=int(input_periods.size), periods_computed=int(computed_periods),
-                     close_root_risk_traces=int(np.count_nonzero(flags & TRACE_CLOSE_ROOT_RISK)))
+        self._record_stats(stats, backend, wave, flags, plan, per.size, chunks, capacity, allocated, clock)
+        result = DispersionBatch(
+            velocity, per, nmodes, wave, flags, levels, counts, bnd, backend, reason, accuracy=self.accuracy,
+            close_root_rtol=self.close_root_rtol, search_strategy=self.search_strategy,
+            sampling_rtol=self.sampling_rtol if self.search_strategy == "shape_adaptive" else None,
+            secular="dunkin" if uniform else (self.secular or "fast"), refinement=self.refinement,
+            love_strategy=self.love_strategy if wave == "love" else None, group_velocity=group,
+            group_velocity_flags=gflags, prediction_flags=prediction,
+            group_rtol=self.group_rtol if self.group_velocity else None, frequency_strategy=self.frequency_strategy,
+            frequency_strategy_used="anchor_continuation" if plan.continuation else "independent",
+            frequency_strategy_reason=plan.reason, anchor_stride=self.anchor_stride, continuity_flags=continuity,
+            continuity_unresolved=stats.get("continuity_unresolved", 0) > 0)
+        self._key = (B, L, T, self.nc, int(nmodes))
+        self._calls += 1
+        self.last_result = result
+        return result
+
+    def _continuity(self, models, plan, wave, bounds, out, prediction, stats):
+        """Screen adjacent periods of approximate presets and review suspects in FP64."""
+        if not _PRESETS[self.accuracy].approximate:
+            return None
+        if not (self.continuity_check or plan.continuation or self.accuracy == "auto"):
+            return None
+        flags, suspect = continuity_screen(plan.grid, out.velocity, prediction, self.continuity_check)
+        if self.accuracy == "auto":
+            failed = (out.flags & (TRACE_UNRESOLVED_BRACKET | TRACE_SEARCH_UNCONVERGED | TRACE_ROOT_CONFLICT)) != 0
+            if failed.any():
+                flags[failed] |= int(ContinuityFlag.SEARCH)
+                suspect = True
+        for k in ("suspected", "local_searches", "full_searches", "repaired", "unresolved", "evaluations"):
+            stats["continuity_" + k] = 0
+        if suspect:
+            _review(self, models, plan.grid, wave, bounds, out.velocity, out.flags, out.levels, out.counts, out.group,
+                    out.group_flags, flags, stats)
+        return flags
+
+    def _record_stats(self, stats, backend, wave, flags, plan, requested, chunks, capacity, allocated, clock):
+        uniform = self.search_strategy == "uniform"
         stats["evaluations"] = sum(int(stats.get(k, 0)) for k in ("scan_nodes", "fp64_rechecks", "refine_evaluations",
                                                                   "probe_evaluations", "probe_evaluations_fp64"))
         attempts = stats.get("continuation_accepted", 0) + stats.get("continuation_fallbacks", 0)
-        stats["continuation_local_success_rate"] = (stats.get("continuation_accepted", 0) / attempts
-                                                    if attempts else None)
+        stats.update(
+            backend=backend, precision=self.precision, accuracy=self.accuracy, requested_strategy=self.search_strategy,
+            strategy="uniform" if uniform else "rocm_exhaustive_scan", search_grid="uniform" if uniform else "adaptive",
+            sampling_rtol=self.sampling_rtol, secular="dunkin" if uniform else (self.secular or "fast"),
+            secular_kernel="sh" if wave == "love" else "dunkin_fast", refinement=self.refinement,
+            love_strategy=self.love_strategy if wave == "love" else None, close_root_rtol=self.close_root_rtol,
+            refinement_tolerance=self.c_tol_rel, chunks=chunks, capacity_models=int(capacity),
+            engine_allocations=int(allocated), group_velocity=self.group_velocity, group_rtol=self.group_rtol,
+            frequency_strategy=self.frequency_strategy,
+            frequency_strategy_used="anchor_continuation" if plan.continuation else "independent",
+            frequency_strategy_reason=plan.reason, anchor_stride=self.anchor_stride,
+            periods_requested=int(requested), periods_computed=int(plan.grid.size),
+            approximate_search_enabled=_PRESETS[self.accuracy].approximate,
+            approximate_traces=int(np.count_nonzero(flags & TRACE_APPROXIMATE_SEARCH)),
+            close_root_risk_traces=int(np.count_nonzero(flags & TRACE_CLOSE_ROOT_RISK)),
+            continuation_local_success_rate=stats.get("continuation_accepted", 0) / attempts if attempts else None,
+            continuity_check=self.continuity_check and _PRESETS[self.accuracy].approximate,
+            workspace_budget_bytes=self.workspace_bytes)
         if self.profile:
-            stats["timings_s"] = {"solver_total": time.perf_counter() - started}
+            stats["timings_s"] = {"solver_total": time.perf_counter() - clock}
         self._last_stats = stats
-        self._calls += 1
-        result = DispersionBatch(velocity, input_periods, nmodes, wave, flags, levels, counts, bnd, backend, reason,
-                                 accuracy=self.accuracy, close_root_rtol=self.close_root_rtol,
-                                 group_velocity=group, group_velocity_flags=gflags,
-                                 prediction_flags=prediction,
-                                 group_rtol=self.group_rtol if self.group_velocity else None,
-                                 frequency_strategy=self.frequency_strategy, frequency_strategy_used=frequency_used,
-                                 frequency_strategy_reason=frequency_reason, anchor_stride=self.anchor_stride,
-                                 continuity_flags=continuity,
-                                 continuity_unresolved=stats.get("continuity_unresolved", 0) > 0)
-        self._key = (B, L, per.size, self.nc, int(nmodes))
-        self.last_result = result
-        return result
 
     @property
     def calls(self):
@@ -497,21 +617,21 @@ class Solver:
         from . import qe_kernel_report, love_phase_kernel_report
         last = self.last_result
         precision = last.precision if last is not None else self.precision
-        report = dict(backend="rocm", precision=precision, accuracy=self.accuracy, secular=self.secular,
-                      secular_kernel=("sh" if last is not None and last.wave == "love" else "dunkin"),
-                      close_root_rtol=self.close_root_rtol, refinement_tolerance=self.c_tol_rel,
-                      refinement=self.refinement, love_strategy=self.love_strategy,
-                      group_velocity=self.group_velocity, group_rtol=self.group_rtol,
+        love = last is not None and last.wave == "love"
+        report = dict(backend="rocm", precision=precision, accuracy=self.accuracy, secular=self.secular or "fast",
+                      secular_kernel="sh" if love else "dunkin_fast", close_root_rtol=self.close_root_rtol,
+                      refinement_tolerance=self.c_tol_rel, refinement=self.refinement,
+                      love_strategy=self.love_strategy, group_velocity=self.group_velocity, group_rtol=self.group_rtol,
                       frequency_strategy=self.frequency_strategy, anchor_stride=self.anchor_stride)
         if last is not None:
             report.update(frequency_strategy_used=last.frequency_strategy_used,
                           frequency_strategy_reason=last.frequency_strategy_reason)
         if self._key is not None:
             cfg = self._engine.config if self._engine is not None else None
-            # The engine scans in FP32 for both output precisions.
             report.update(qe_kernel_report(self._key[1], int(self.accuracy != "high"),
-                                           self.search_strategy == "shape_adaptive", True, config=cfg))
-            if last is not None and last.wave == "love":
+                                           self.search_strategy == "shape_adaptive", precision == "float32",
+                                           config=cfg))
+            if love and self.love_strategy == "phase":
                 report["scan_kernel_report"] = dict(report)
                 report.update(love_phase_kernel_report(self._key[1], precision == "float32"))
                 report["scan_kernel"] = report.get("search_kernel")
@@ -524,135 +644,154 @@ class Solver:
         return dict(self._last_stats)
 
 
-def _review(solver, models, periods, wave, bounds, velocity, trace, levels, counts, group, group_flags, flags,
-            stats):
-    """FP64 review of the traces the continuity screen flagged (reference
-    python/modisp/_continuity.py:17-205, its row-period path): every suspect
-    (model, period) is re-solved alone by an accuracy="high" solver with the
-    requested modes plus two guard roots -- first in a window cut just above
-    the observed roots, then, where that is not conclusive, in the full window
-    -- and adopted when the verifier's search is clean; a changed result
-    replaces the trace (REPAIRED), one the verifier cannot confirm is marked
-    UNRESOLVED, and every suspect is REVIEWED."""
-    reason_bits = int(ContinuityFlag.ENVELOPE | ContinuityFlag.CLOSE_FP32 | ContinuityFlag.PREDICTION
-                      | ContinuityFlag.SEARCH)
-    model_ids, freq_ids = np.nonzero(flags & reason_bits)
-    n = len(model_ids)
+# --- continuity review -----------------------------------------------------------------------
+_REASONS = int(ContinuityFlag.ENVELOPE | ContinuityFlag.CLOSE_FP32 | ContinuityFlag.PREDICTION | ContinuityFlag.SEARCH)
+_VERIFIER_FAILURE = TRACE_SEARCH_UNCONVERGED | TRACE_UNRESOLVED_BRACKET | TRACE_F_REJECTED
+
+
+def _review(solver, models, periods, wave, bounds, velocity, trace, levels, counts, group, group_flags, flags, stats):
+    """FP64 review of the traces the continuity screen flagged (the reference
+    semantics are python/modisp/_continuity.py:17-205): each suspect is
+    re-solved alone at accuracy="high" with two guard roots beyond the
+    requested modes, first in a window cut just above its observed roots when
+    the trace showed both guards, then in the full window. A verified result
+    that differs from the reported one replaces it (REPAIRED), one that cannot
+    be verified leaves the trace UNRESOLVED, and every suspect is REVIEWED.
+    The arrays are updated in place."""
+    rows, cols = np.nonzero(flags & _REASONS)
+    n = rows.size
     stats["continuity_suspected"] = n
     if not n:
         return
     modes = velocity.shape[1]
     guards = min(256, modes + 2)
-    settings = dict(backend="rocm", device=solver.device, accuracy="high", cmin=solver.cmin, cmax=solver.cmax,
-                    c_tol_rel=min(solver.c_tol_rel, 1e-6), f_accept=solver.f_accept, secular=solver.secular,
-                    refinement=solver.refinement, group_velocity=solver.group_velocity, group_rtol=solver.group_rtol,
-                    close_root_rtol=solver.close_root_rtol, continuity_check=False,
-                    workspace_bytes=solver.workspace_bytes, chunk_size=solver.chunk_size)
-    with Solver(**settings) as verifier:
-        def solve(rows, frequencies, limits, local):
-            selected = periods[frequencies]
-            stats["continuity_" + ("local_searches" if local else "full_searches")] += len(rows)
-            try:
-                result = verifier._solve(*(v[rows] for v in models), selected[:1], guards, wave, _bounds=limits,
-                                         _row_periods=selected)
-            except MemoryError:
-                return None
-            counters = verifier.stats()
-            stats["continuity_rocm_batches"] = stats.get("continuity_rocm_batches", 0) + counters["chunks"]
-            stats["continuity_evaluations"] += counters.get("evaluations", 0)
-            return result
-
-        def adopt(result, i, row, f, local):
-            if result is None:
-                return False
-            count = int(result.root_count[i, 0])
-            roots = result.phase_velocity[i, :, 0]
-            known = min(guards, count)
-            if (count < 0 or result.search_level[i, 0] < 0 or result.trace_flags[i, 0]
-                    & (TRACE_SEARCH_UNCONVERGED | TRACE_UNRESOLVED_BRACKET | TRACE_F_REJECTED)):
-                return False
-            if local and count < modes + 2:
-                return False
-            recovered = roots[:known].astype(velocity.dtype)
-            if not np.isfinite(recovered).all() or np.any(recovered[1:] <= recovered[:-1]):
-                return False
-            if local and (not known or roots[known - 1] >= result.search_bounds[i, 1] * (1 - 4 * verifier.c_tol_rel)):
-                return False
-            old = velocity[row, :, f]
-            new = roots[:modes].astype(old.dtype)
-            count = min(count, modes + 2)
-            alias_changed = bool(flags[row, f] & int(ContinuityFlag.CLOSE_FP32)) and not np.array_equal(
-                old, new, equal_nan=True)
-            search_failed = bool(flags[row, f] & int(ContinuityFlag.SEARCH))
-            if (search_failed or counts[row, f] != count or alias_changed
-                    or not np.allclose(old, new, rtol=solver.c_tol_rel, atol=0, equal_nan=True)):
-                old[:] = new
-                counts[row, f] = count
-                word = int(result.trace_flags[i, 0])
-                cutoff = .99 * models[2][row, -1]
-                if (word & TRACE_NEAR_CUTOFF and np.any(roots[modes:] >= cutoff)
-                        and not np.any(roots[:modes] >= cutoff)):
-                    word &= ~TRACE_NEAR_CUTOFF
-                trace[row, f] = word
-                if old.dtype == np.float32:
-                    trace[row, f] |= TRACE_FP64_REPAIR
-                levels[row, f] = result.search_level[i, 0]
-                if group is not None:
-                    group[row, :, f] = result.group_velocity[i, :modes, 0]
-                    group_flags[row, :, f] = result.group_velocity_flags[i, :modes, 0]
-                flags[row, f] |= int(ContinuityFlag.REPAIRED)
-                stats["continuity_repaired"] += 1
-            return True
-
-        # Local window: keep the original lower bound (rank) and cut the upper
-        # bound just above the observed roots when two guards were found.
-        roots = velocity[model_ids, :, freq_ids][:, :, None]
-        observed = np.where(np.isfinite(roots), roots, 0.).max(axis=(1, 2))
-        if modes > 1:
-            gaps = np.diff(roots, axis=1)
-            margin = np.where(np.isfinite(gaps), gaps, 0.).max(axis=(1, 2))
-        else:
-            margin = .1 * observed
-        limits = bounds[model_ids]
-        upper = observed + np.maximum(2.5 * margin, .02 * observed)
-        have_guards = counts[model_ids, freq_ids] >= modes + 2
-        eligible = have_guards & (guards == modes + 2) & (upper > limits[:, 0]) & (upper < limits[:, 1] * .98)
-        pending = np.ones(n, dtype=bool)
-        for local in (True, False):
-            ids = np.flatnonzero(eligible if local else pending)
-            if not ids.size:
+    limits = bounds[rows].copy()
+    # Local windows: keep the lower bound (it fixes the modes' ranks) and cut
+    # the upper bound above the highest observed root by 2.5 of the largest
+    # observed gap (or 2% of the root), if both guards were found.
+    seen = velocity[rows, :, cols].astype(np.float64)
+    top = np.nanmax(np.where(np.isfinite(seen), seen, 0.), axis=1)
+    if modes > 1:
+        gaps = np.diff(seen, axis=1)
+        margin = np.nanmax(np.where(np.isfinite(gaps), gaps, 0.), axis=1)
+    else:
+        margin = .1 * top
+    cut = top + np.maximum(2.5 * margin, .02 * top)
+    local = ((counts[rows, cols] >= modes + 2) & (guards == modes + 2) & (cut > limits[:, 0])
+             & (cut < .98 * limits[:, 1]))
+    settled = np.zeros(n, dtype=bool)
+    verifier_options = dict(backend="rocm", device=solver.device, accuracy="high", cmin=solver.cmin,
+                            cmax=solver.cmax, c_tol_rel=min(solver.c_tol_rel, 1e-6), f_accept=solver.f_accept,
+                            secular=solver.secular, refinement=solver.refinement,
+                            group_velocity=solver.group_velocity, group_rtol=solver.group_rtol,
+                            close_root_rtol=solver.close_root_rtol, continuity_check=False,
+                            workspace_bytes=solver.workspace_bytes, chunk_size=solver.chunk_size)
+    with Solver(**verifier_options) as verifier:
+        for stage, members in (("local", local), ("full", None)):
+            idx = np.flatnonzero(members if members is not None else ~settled)
+            if not idx.size:
                 continue
-            cut = limits[ids].copy()
-            if local:
-                cut[:, 1] = upper[ids]
-            result = solve(model_ids[ids], freq_ids[ids], cut, local)
-            for i, k in enumerate(ids):
-                pending[k] = not adopt(result, i, model_ids[k], freq_ids[k], local)
-        flags[model_ids[pending], freq_ids[pending]] |= int(ContinuityFlag.UNRESOLVED)
-        stats["continuity_unresolved"] += int(np.count_nonzero(pending))
-    flags[model_ids, freq_ids] |= int(ContinuityFlag.REVIEWED)
+            window = limits[idx].copy()
+            if stage == "local":
+                window[:, 1] = cut[idx]
+            stats[f"continuity_{stage}_searches"] += idx.size
+            try:
+                res = verifier._solve(*(m[rows[idx]] for m in models), periods[cols[idx]][:1], guards, wave,
+                                      _bounds=window, _row_periods=periods[cols[idx]])
+            except MemoryError:
+                continue
+            stats["continuity_evaluations"] += verifier.stats().get("evaluations", 0)
+            ok = _verified(res, stage == "local", modes, guards, velocity.dtype, verifier.c_tol_rel)
+            _adopt(res, idx[ok], np.flatnonzero(ok), rows, cols, models, velocity, trace, levels, counts, group,
+                   group_flags, flags, solver.c_tol_rel, stats)
+            settled[idx[ok]] = True
+    flags[rows[~settled], cols[~settled]] |= int(ContinuityFlag.UNRESOLVED)
+    stats["continuity_unresolved"] += int(np.count_nonzero(~settled))
+    flags[rows, cols] |= int(ContinuityFlag.REVIEWED)
 
 
+def _verified(res, local, modes, guards, dtype, tol):
+    """Which verifier rows are conclusive: a clean FP64 search whose known
+    roots (guards included) are finite and strictly ascending in the result
+    precision; a local search must also have found both guards below its cut."""
+    count = res.root_count[:, 0].astype(np.int64)
+    ok = (count >= 0) & (res.search_level[:, 0] >= 0) & ((res.trace_flags[:, 0] & _VERIFIER_FAILURE) == 0)
+    if local:
+        ok &= count >= modes + 2
+    known = np.minimum(count, guards)
+    roots = res.phase_velocity[:, :, 0].astype(dtype)
+    inside = np.arange(roots.shape[1])[None, :] < known[:, None]
+    finite = np.where(inside, np.isfinite(roots), True).all(axis=1)
+    pairs = inside[:, 1:]
+    rising = np.where(pairs, roots[:, 1:] > roots[:, :-1], True).all(axis=1)
+    ok &= finite & rising
+    if local:
+        last = np.take_along_axis(res.phase_velocity[:, :, 0], np.maximum(known - 1, 0)[:, None], 1)[:, 0]
+        ok &= (known > 0) & (last < res.search_bounds[:, 1] * (1 - 4 * tol))
+    return ok
+
+
+def _adopt(res, members, res_rows, rows, cols, models, velocity, trace, levels, counts, group, group_flags, flags, tol,
+           stats):
+    """Write the verified results that differ from the reported ones."""
+    if not members.size:
+        return
+    modes = velocity.shape[1]
+    r, c = rows[members], cols[members]
+    new = res.phase_velocity[res_rows, :modes, 0].astype(velocity.dtype)
+    old = velocity[r, :, c]
+    count = np.minimum(res.root_count[res_rows, 0], modes + 2)
+    same = np.isclose(old, new, rtol=tol, atol=0, equal_nan=True).all(axis=1)
+    identical = ((old == new) | (np.isnan(old) & np.isnan(new))).all(axis=1)
+    reason = flags[r, c]
+    change = (((reason & int(ContinuityFlag.SEARCH)) != 0) | (counts[r, c] != count) | ~same
+              | (((reason & int(ContinuityFlag.CLOSE_FP32)) != 0) & ~identical))
+    if not change.any():
+        return
+    k = np.flatnonzero(change)
+    r, c, rr = r[k], c[k], res_rows[k]
+    velocity[r, :, c] = new[k]
+    counts[r, c] = count[k]
+    word = res.trace_flags[rr, 0].copy()
+    # A cutoff flag raised only by a guard root does not describe the reported modes.
+    cutoff = .99 * models[2][r, -1]
+    every = res.phase_velocity[rr, :, 0]
+    guard_hit = np.any(np.nan_to_num(every[:, modes:], nan=-np.inf) >= cutoff[:, None], axis=1)
+    mode_hit = np.any(np.nan_to_num(every[:, :modes], nan=-np.inf) >= cutoff[:, None], axis=1)
+    word = np.where(((word & TRACE_NEAR_CUTOFF) != 0) & guard_hit & ~mode_hit, word & ~np.uint32(TRACE_NEAR_CUTOFF),
+                    word)
+    if velocity.dtype == np.float32:
+        word |= np.uint32(TRACE_FP64_REPAIR)
+    trace[r, c] = word
+    levels[r, c] = res.search_level[rr, 0]
+    if group is not None:
+        group[r, :, c] = res.group_velocity[rr, :modes, 0]
+        group_flags[r, :, c] = res.group_velocity_flags[rr, :modes, 0]
+    flags[r, c] |= int(ContinuityFlag.REPAIRED)
+    stats["continuity_repaired"] += int(k.size)
+
+
+# --- convenience functions -------------------------------------------------------------------
 def dispersion_batch(thickness, vp, vs, rho, periods, nmodes=5, wave="rayleigh", solver=None, names=None,
                      search_strategy=None, accuracy=None, **options):
-    """One call on a batch [B, L] (reference __init__.py:572-589)."""
+    """One call on a batch [B, L]; with `solver`, no other option may differ from it."""
     if solver is not None:
-        if (options or (search_strategy is not None and search_strategy != solver.search_strategy)
-                or (accuracy is not None and accuracy != solver.accuracy)):
+        conflicting = (options or (search_strategy is not None and search_strategy != solver.search_strategy)
+                       or (accuracy is not None and accuracy != solver.accuracy
+                           and not (accuracy == "auto" and solver.search_strategy == "uniform")))
+        if conflicting:
             raise ValueError("options conflict with the supplied Solver")
         return solver(thickness, vp, vs, rho, periods, nmodes, wave, names)
-    with Solver(search_strategy="shape_adaptive" if search_strategy is None else search_strategy,
-                accuracy=accuracy, **options) as s:
+    with Solver(search_strategy=search_strategy or "shape_adaptive", accuracy=accuracy, **options) as s:
         return s(thickness, vp, vs, rho, periods, nmodes, wave, names)
 
 
 def dispersion(model, periods, nmodes=5, wave="rayleigh", **options):
-    """One model given as an [L, 4] array of columns h, vp, vs, rho or a dict
-    of the four arrays (reference __init__.py:591-597)."""
+    """One model: an [L, 4] array of columns h, vp, vs, rho, or a dict of the four arrays."""
     batch = model
     if not isinstance(batch, dict):
         arr = np.asarray(batch)
         if arr.ndim != 2 or arr.shape[1] != 4:
-            raise ValueError("model must be an array with columns h/vp/vs/rho, or a batch dict")
-        batch = dict(zip(("thickness", "vp", "vs", "rho"), arr.T))
+            raise ValueError("model must be LayeredModel, an array with columns h/vp/vs/rho, or a batch dict")
+        batch = {k: arr[:, j] for j, k in enumerate(("thickness", "vp", "vs", "rho"))}
     return dispersion_batch(**batch, periods=periods, nmodes=nmodes, wave=wave, **options)

@@ -1497,6 +1497,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   }
 }
 
+// Group velocities of one confirmed trace from its ordered roots (the
+// continuity review's helper, reference cpp/common/review_groups.hpp:10-31):
+// lane m takes reported mode m; modes beyond the known roots are missing.
+__global__ void review_group_kernel(const PrepRow *rows, int L, int top, int wave, double omega, const double *roots,
+                                    int known, int modes, double lower, double upper, double close_rtol,
+                                    double group_rtol, double *out, uint32_t *flags) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= modes) return;
+  unsigned f = kGroupMissing;
+  double u = NAN;
+  if (m < known) u = group_result(rows, L, top, wave, omega, roots, m, known, lower, upper, close_rtol, group_rtol, &f);
+  out[m] = u;
+  flags[m] = f;
+}
+
+hipError_t launch_review_groups(const PrepRow *rows, int L, int top, int wave, double omega, const double *roots,
+                                int known, int modes, double lower, double upper, double close_rtol,
+                                double group_rtol, double *out, uint32_t *flags, hipStream_t s) {
+  hipLaunchKernelGGL(review_group_kernel, dim3((modes + 63) / 64), dim3(64), 0, s, rows, L, top, wave, omega, roots,
+                     known, modes, lower, upper, close_rtol, group_rtol, out, flags);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Anchor continuation (the step rules follow cpp/common/continuation.hpp:12-212).
 // ---------------------------------------------------------------------------

@@ -228,6 +228,9 @@ struct ContArgs {
   int *fallback_count;
 };
 hipError_t launch_continuation(const ContArgs &c, int persistent_blocks, hipStream_t s);
+hipError_t launch_review_groups(const PrepRow *rows, int L, int top, int wave, double omega, const double *roots,
+                                int known, int modes, double lower, double upper, double close_rtol,
+                                double group_rtol, double *out, uint32_t *flags, hipStream_t s);
 hipError_t launch_gather(const TraceInfo *all, const int *list, int n, TraceInfo *out, hipStream_t s);
 using ScanArgs = PipeArgs;
 

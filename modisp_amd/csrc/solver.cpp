@@ -550,6 +550,43 @@ int md_rocm_device_name(int device, char *buf, int len) {
   return 0;
 }
 
+int md_rocm_review_groups(const double *packed, size_t size, int layers, int modes, int wave, int secular,
+                          double period, double lower, double upper, double group_rtol, double close_rtol,
+                          double *group, uint32_t *flags, int *evaluations) {
+  if (!packed || !group || !flags || layers < 1 || layers > mdr::kMaxRows || modes < 1 || modes > 256 ||
+      size < static_cast<size_t>(4) * layers || size > static_cast<size_t>(4) * layers + 256 || !(period > 0.) ||
+      (wave != 0 && wave != 1))
+    return fail(2, "invalid review trace");
+  if (secular != 1) return fail(2, "secular must be 1 (fast): the only evaluator built on this backend");
+  const int known = static_cast<int>(size - static_cast<size_t>(4) * layers);
+  const double *h = packed, *vp = packed + layers, *vs = packed + 2 * layers, *rho = packed + 3 * layers;
+  std::vector<mdr::PrepRow> rows(layers);
+  for (int j = 0; j < layers; ++j) rows[j] = mdr::make_prep_row(h[j], vp[j], vs[j], rho[j]);
+  const size_t bytes = layers * sizeof(mdr::PrepRow) + (known + 2 * modes) * sizeof(double) + modes * 4 + 256;
+  DevBuf buf;
+  if (int e = buf.ensure(bytes)) return e;
+  auto *base = static_cast<unsigned char *>(buf.p);
+  auto *d_rows = reinterpret_cast<mdr::PrepRow *>(base);
+  auto *d_roots = reinterpret_cast<double *>(base + layers * sizeof(mdr::PrepRow));
+  auto *d_group = d_roots + std::max(known, 1);
+  auto *d_flags = reinterpret_cast<uint32_t *>(d_group + modes);
+  MD_HIP(hipMemcpy(d_rows, rows.data(), layers * sizeof(mdr::PrepRow), hipMemcpyHostToDevice));
+  if (known) MD_HIP(hipMemcpy(d_roots, packed + 4 * layers, known * sizeof(double), hipMemcpyHostToDevice));
+  const int top = vs[0] == 0. ? 1 : 0;
+  constexpr double two_pi = 6.283185307179586;
+  MD_HIP(mdr::launch_review_groups(d_rows, layers, top, wave, two_pi / period, d_roots, known, modes, lower, upper,
+                                   close_rtol, group_rtol, d_group, d_flags, nullptr));
+  MD_HIP(hipMemcpy(group, d_group, modes * sizeof(double), hipMemcpyDeviceToHost));
+  MD_HIP(hipMemcpy(flags, d_flags, modes * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (evaluations) {  // one dual-number evaluation per root that reached the derivative
+    int n = 0;
+    for (int m = 0; m < modes; ++m) n += m < known && flags[m] != mdr::kGroupMissing && flags[m] != mdr::kGroupClose &&
+                                         flags[m] != mdr::kGroupPolish;
+    *evaluations = n;
+  }
+  return 0;
+}
+
 static int fill_report(int which, int L, int single_precision, const md_rocm_config *cfg, md_rocm_kernel_report *out) {
   if (L < 1 || L > mdr::kMaxRows) return fail(2, "L must be 1..256");
   if (!out) return fail(2, "null report");

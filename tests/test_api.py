@@ -270,3 +270,144 @@ def test_result_masks_equal_the_reference_numpy_expressions(dtype, modes, rtol):
         assert want[2].any() and not want[2].all()      # the threshold cases are exercised
         assert split > 100 or not (dtype == np.float32 and rtol == .007)  # FP32 cells FP64 would flip
     assert (want[0] == api.SolveStatus.REJECTED).any() and (want[0] == api.SolveStatus.UNRESOLVED).any()
+
+
+# --- options that select methods (CPU: validation; GPU: effect) -------------------------------
+def test_uniform_strategy_follows_the_reference_restrictions():
+    with pytest.raises(NotImplementedError, match="legacy refinement"):
+        api.Solver(search_strategy="uniform", refinement="brent")
+    with pytest.raises(NotImplementedError, match="adaptive solver"):
+        api.Solver(search_strategy="uniform", group_velocity=True)
+    with pytest.raises(ValueError, match="float32"):
+        api.Solver(search_strategy="uniform", precision="float32")
+    s = api.Solver(search_strategy="uniform")
+    assert s.accuracy == "uniform" and s.refinement == "legacy" and s.c_tol_rel == 1e-6
+
+
+def test_engine_options_follow_the_front_end_options():
+    s = api.Solver(accuracy="balanced", search_strategy="qe_adaptive", love_strategy="scan", precision="float32",
+                   refinement="chandrupatla")
+    o = s._engine_options(False)
+    assert (o["accuracy"], o["shape"], o["love_phase"], o["single_precision"], o["refinement"]) == (1, False, 0, True, 2)
+    o = api.Solver()._engine_options(False)           # the reference defaults: auto, shape_adaptive, phase, FP64
+    assert (o["accuracy"], o["shape"], o["love_phase"], o["single_precision"], o["sampling_rtol"]) == (4, True, 1,
+                                                                                                      False, -1.)
+    assert api.Solver(secular="dunkin")._engine_options(False)["secular"] == 0    # refused by the engine
+
+
+def test_review_groups_validates_its_trace():
+    with pytest.raises(ValueError, match="invalid review trace"):
+        modisp_amd._review_groups(np.ones(7), 2, 3, 0, 1, 1., 1., 2., 1e-3, .005)   # fewer than 4 layers of data
+    with pytest.raises(ValueError, match="invalid review trace"):
+        modisp_amd._review_groups(np.ones(8), 2, 3, 0, 1, -1., 1., 2., 1e-3, .005)  # period
+
+
+@pytest.mark.gpu
+def test_precision_float64_runs_the_plain_fp64_scan(gpu):
+    wl = workloads.plain_batch(B=4, seed=9)
+    with api.Solver(accuracy="high", c_tol_rel=1e-12) as s:
+        r64 = s(wl.thickness, wl.vp, wl.vs, wl.rho, wl.periods[::10], 3)
+        assert s._engine.scan == "float64" and s._engine.config.single_precision == 0
+        assert s.stats()["fp64_rescans"] == 0 and s.stats()["fp64_rechecks"] == 0
+    with api.Solver(accuracy="high", c_tol_rel=1e-12, precision="float32") as s:
+        r32 = s(wl.thickness, wl.vp, wl.vs, wl.rho, wl.periods[::10], 3)
+        assert s._engine.scan == "float32"
+    assert r64.phase_velocity.dtype == np.float64 and r32.phase_velocity.dtype == np.float32
+    np.testing.assert_array_equal(r64.mode_mask, r32.mode_mask)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("accept", [1e-30, 1e-3])
+def test_f_accept_rejects_like_the_reference(gpu, accept):
+    """f_accept < 1 rejects roots whose |F| exceeds it: NaN, TRACE_F_REJECTED,
+    status REJECTED, group flag MISSING_ROOT (reference cpp/cpu/solver.cpp:333).
+    1e-30 rejects every root (|F| at a root is ~1e-16 or more), 1e-3 none."""
+    import oracle
+    wl = workloads.plain_batch(B=6, seed=10)
+    per = wl.periods[::8]
+    with api.Solver(accuracy="high", c_tol_rel=1e-12, f_accept=accept, group_velocity=True) as s:
+        got = s(wl.thickness, wl.vp, wl.vs, wl.rho, per, 3)
+    b = oracle.bounds(wl.thickness, wl.vp, wl.vs, wl.rho, 0)
+    ref = oracle.solve(wl.thickness, wl.vp, wl.vs, wl.rho, per, b, 3, 0, automatic=True, tol=1e-12, accept=accept)
+    # Where the two evaluators' |F| at a root falls on either side of the
+    # threshold only by rounding (|F| of the oracle at the device root within
+    # 1e-14 of zero: one of them evaluates exactly 0), the verdicts may differ.
+    differ = got.mode_mask != np.isfinite(ref["phase_velocity"])
+    for i, m, f in np.argwhere(differ):
+        c = got.phase_velocity[i, m, f] if got.mode_mask[i, m, f] else ref["phase_velocity"][i, m, f]
+        model = [getattr(wl, k)[i] for k in ("thickness", "vp", "vs", "rho")]
+        assert abs(oracle.secular(*model, per[f], c)[0]) <= 1e-14, (i, m, f)
+    same = ~differ.any(axis=1)
+    np.testing.assert_array_equal((got.trace_flags & api.TRACE_F_REJECTED)[same],
+                                  (ref["trace_flags"] & api.TRACE_F_REJECTED)[same])
+    assert differ.sum() <= 2
+    if accept < 1e-20:
+        assert got.mode_mask.sum() <= differ.sum()
+        found = np.arange(3)[None, :, None] < got.root_count[:, None, :]
+        assert np.all((got.status == api.SolveStatus.REJECTED) == (found & ~got.mode_mask))
+        assert np.all(got.group_velocity_flags[got.status == api.SolveStatus.REJECTED] == 1)
+    else:
+        assert not (got.trace_flags & api.TRACE_F_REJECTED).any()
+
+
+@pytest.mark.gpu
+def test_presets_report_the_reference_levels_and_approximate_flags(gpu):
+    wl = workloads.plain_batch(B=3, seed=11)
+    per = wl.periods[::12]
+    cases = (("high", "shape_adaptive", 8, 0), ("balanced", "shape_adaptive", 9, api.TRACE_APPROXIMATE_SEARCH),
+             ("fast", "shape_adaptive", 10, api.TRACE_APPROXIMATE_SEARCH), ("high", "qe_adaptive", 3, 0),
+             ("balanced", "qe_adaptive", 6, api.TRACE_APPROXIMATE_SEARCH),
+             ("fast", "qe_adaptive", 7, api.TRACE_APPROXIMATE_SEARCH))
+    for accuracy, strategy, level, flag in cases:
+        with api.Solver(accuracy=accuracy, search_strategy=strategy, continuity_check=False) as s:
+            r = s(wl.thickness, wl.vp, wl.vs, wl.rho, per, 2)
+        assert np.all(r.search_level == level), (accuracy, strategy)
+        assert np.all((r.trace_flags & api.TRACE_APPROXIMATE_SEARCH) == flag), (accuracy, strategy)
+    with api.Solver(accuracy="high", love_strategy="scan") as s:
+        love = s(wl.thickness, wl.vp, wl.vs, wl.rho, per, 2, "love")
+    assert not np.any(love.search_level == 11)
+    with api.Solver(accuracy="high") as s:
+        love = s(wl.thickness, wl.vp, wl.vs, wl.rho, per, 2, "love")
+    # The phase count accepts a trace (11) unless the unwrapped phase is lost
+    # at a window end (an opaque evanescent stack at the automatic window's
+    # low end); then the scan's roots stand with TRACE_LOVE_PHASE_FALLBACK, as
+    # in the reference (cpp/cpu/solver.cpp:285-291).
+    fallback = (love.trace_flags & api.TRACE_LOVE_PHASE_FALLBACK) != 0
+    assert np.all(love.search_level[~fallback] == 11) and np.all(love.search_level[fallback] == 8)
+    assert (~fallback).any()
+
+
+@pytest.mark.gpu
+def test_uniform_strategy_runs_the_uniform_grid(gpu):
+    import oracle
+    wl = workloads.plain_batch(B=4, seed=12)
+    per = wl.periods[::10]
+    with api.Solver(search_strategy="uniform", nc=1024, c_tol_rel=1e-12) as s:
+        r = s(wl.thickness, wl.vp, wl.vs, wl.rho, per, 3)
+        assert s.stats()["search_grid"] == "uniform"
+    assert np.all(r.search_level == -2) and np.all(r.trace_flags & api.TRACE_RANGE_TRUNCATED)
+    assert np.all(r.search_bounds[:, 0] == .2)
+    ref = oracle.solve(wl.thickness, wl.vp, wl.vs, wl.rho, per, r.search_bounds, 3, 0, tol=1e-12)
+    ok = r.mode_mask & np.isfinite(ref["phase_velocity"])
+    assert ok.sum() > .9 * r.mode_mask.sum()
+    np.testing.assert_allclose(r.phase_velocity[ok], ref["phase_velocity"][ok], rtol=1e-9)
+
+
+@pytest.mark.gpu
+def test_review_groups_equal_the_oracle_analytic_group(gpu):
+    import oracle
+    wl = workloads.precision_batch(B=2, seed=13)
+    h, vp, vs, rho = (getattr(wl, k)[0] for k in ("thickness", "vp", "vs", "rho"))
+    win = workloads.scan_window(wl.vs)[0]
+    period = 3.
+    ref = oracle.solve(h, vp, vs, rho, [period], win[None], 6, 0, tol=1e-13, group=2)
+    roots = ref["phase_velocity"][0, :, 0]
+    roots = roots[np.isfinite(roots)]
+    packed = np.concatenate([h, vp, vs, rho, roots])
+    u, f, ev = modisp_amd._review_groups(packed, h.size, 4, 0, 1, period, win[0], win[1], 1e-3, .005)
+    np.testing.assert_array_equal(f, ref["group_velocity_flags"][0, :4, 0])
+    ok = f == 0
+    np.testing.assert_allclose(u[ok], ref["group_velocity"][0, :4, 0][ok], rtol=1e-9)
+    assert ev == int(ok.sum())
+    u2, f2, _ = modisp_amd._review_groups(packed[:4 * h.size + 2], h.size, 4, 0, 1, period, win[0], win[1], 1e-3, .005)
+    assert np.all(f2[2:] == 1) and np.all(np.isnan(u2[2:]))
