@@ -1154,48 +1154,87 @@ extern "C" int md_rocm_probe_grid(const double *thickness, const double *vp, con
   return code;
 }
 
-// Continuity screen of completed phases in caller grid order (reference
-// cpp/cpu/continuity_screen.hpp:8-67, used by python/modisp/__init__.py:446-455):
-// per (model, period) a byte of ContinuityFlag bits -- 8 NO_BILATERAL (no mode
-// has finite values at both neighbouring periods), 16 ENVELOPE (a mode leaves
-// [0.95 min, 1.05 max] of its neighbours, or vanishes between them), 32
-// CLOSE_FP32 (FP32 phases of adjacent modes within 16 ulp), 64 PREDICTION
-// (continuation asked for review). Neighbours are the adjacent periods in
-// ascending order, used only when their log step is at most 0.25.
+// Continuity screen of completed phases in caller grid order (semantics of
+// the reference's cpp/cpu/continuity_screen.hpp:8-67, used by
+// python/modisp/__init__.py:446-455): per (model, period) a byte of
+// ContinuityFlag bits -- 8 NO_BILATERAL (no mode has finite values at both
+// neighbouring periods), 16 ENVELOPE (a mode leaves [0.95 min, 1.05 max] of its
+// neighbours, or vanishes between them), 32 CLOSE_FP32 (FP32 phases of
+// adjacent modes within 16 ulp), 64 PREDICTION (continuation asked for
+// review). Neighbours are the adjacent periods in ascending order, used only
+// when their log step is at most 0.25.
+//
+// The neighbour table (left, right, usable) is built once from the period
+// order; each model is then screened row by row over its contiguous [K, T]
+// block, accumulating per-period masks: "some mode has both neighbours"
+// (clears NO_BILATERAL), "some mode leaves its envelope", "some adjacent
+// modes alias in FP32".
+struct Neighbours {
+  std::vector<int> left, right;
+  std::vector<uint8_t> usable;
+  Neighbours(const double *p, int T, bool bilateral) : left(T), right(T), usable(T, 0) {
+    std::vector<int> rank(T);
+    for (int i = 0; i < T; ++i) rank[i] = i;
+    if (bilateral && T > 2 && !std::is_sorted(p, p + T))
+      std::stable_sort(rank.begin(), rank.end(), [&](int i, int j) { return p[i] < p[j]; });
+    for (int j = 0; j < T; ++j) {
+      const int f = rank[j];
+      left[f] = rank[j > 0 ? j - 1 : j];
+      right[f] = rank[j + 1 < T ? j + 1 : j];
+      usable[f] = bilateral && j > 0 && j + 1 < T && cont_log_step(p[right[f]], p[left[f]]) <= .25;
+    }
+  }
+};
+
+template <class T>
+static bool leaves_envelope(T x, T a, T b) {
+  return !std::isfinite(x) || x < T(.95) * std::min(a, b) || x > T(1.05) * std::max(a, b);
+}
+
+template <class T>
+static bool fp32_alias(T x, T y) {
+  if constexpr (sizeof(T) != 4) {
+    (void)x;
+    (void)y;
+    return false;
+  } else {
+    if (!std::isfinite(x) || !std::isfinite(y)) return false;
+    uint32_t a, c;
+    std::memcpy(&a, &x, 4);
+    std::memcpy(&c, &y, 4);
+    return (a > c ? a - c : c - a) <= 16u;
+  }
+}
+
 template <class T>
 static int screen(const double *p, int Tn, const T *v, int B, int K, const uint32_t *pred, int bilateral,
                   uint8_t *out, int *suspect) {
-  std::vector<int> order;
-  if (bilateral && Tn > 2 && !std::is_sorted(p, p + Tn)) {
-    order.resize(Tn);
-    for (int i = 0; i < Tn; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int i, int j) { return p[i] < p[j]; });
-  }
-  auto at = [&](int rank) { return order.empty() ? rank : order[rank]; };
+  const Neighbours nb(p, Tn, bilateral != 0);
+  const bool aliases = bilateral && sizeof(T) == 4 && K > 1;
+  std::vector<uint8_t> paired(Tn), jump(Tn), alias(Tn);
   int any = 0;
-  for (int j = 0; j < Tn; ++j) {
-    const int f = at(j), l = at(j > 0 ? j - 1 : j), r = at(j + 1 < Tn ? j + 1 : j);
-    const bool interior = bilateral && j > 0 && j + 1 < Tn && cont_log_step(p[r], p[l]) <= .25;
-    const bool fp32_pairs = bilateral && sizeof(T) == 4 && K > 1;
-    for (int b = 0; b < B; ++b) {
-      uint8_t flag = bilateral ? 8 : 0;
-      if (interior || fp32_pairs) {
-        for (int k = 0; k < K; ++k) {
-          const T *mode = v + (static_cast<size_t>(b) * K + k) * Tn;
-          if (interior && std::isfinite(mode[l]) && std::isfinite(mode[r])) {
-            flag &= static_cast<uint8_t>(~8u);
-            if (!std::isfinite(mode[f]) || mode[f] < T(.95) * std::min(mode[l], mode[r]) ||
-                mode[f] > T(1.05) * std::max(mode[l], mode[r]))
-              flag |= 16;
-          }
-          if (sizeof(T) == 4 && k && std::isfinite(mode[f]) && std::isfinite(mode[f - Tn])) {
-            uint32_t x, y;
-            std::memcpy(&x, mode + f, 4);
-            std::memcpy(&y, mode + f - Tn, 4);
-            if (std::max(x, y) - std::min(x, y) <= 16u) flag |= 32;
-          }
-        }
+  for (int b = 0; b < B; ++b) {
+    std::fill(paired.begin(), paired.end(), 0);
+    std::fill(jump.begin(), jump.end(), 0);
+    std::fill(alias.begin(), alias.end(), 0);
+    const T *block = v + static_cast<size_t>(b) * K * Tn;
+    for (int k = 0; k < K; ++k) {
+      const T *row = block + static_cast<size_t>(k) * Tn;
+      for (int f = 0; f < Tn; ++f) {
+        if (!nb.usable[f]) continue;
+        const T a = row[nb.left[f]], c = row[nb.right[f]];
+        if (!std::isfinite(a) || !std::isfinite(c)) continue;
+        paired[f] = 1;
+        jump[f] |= leaves_envelope(row[f], a, c);
       }
+      if (aliases && k > 0)
+        for (int f = 0; f < Tn; ++f) alias[f] |= fp32_alias(row[f], row[f - Tn]);
+    }
+    for (int f = 0; f < Tn; ++f) {
+      uint8_t flag = 0;
+      if (bilateral && !paired[f]) flag |= 8;
+      if (jump[f]) flag |= 16;
+      if (alias[f]) flag |= 32;
       if (pred && (pred[static_cast<size_t>(b) * Tn + f] & mdr::kPredReview)) flag |= 64;
       out[static_cast<size_t>(b) * Tn + f] = flag;
       any |= (flag & (16 | 32 | 64)) != 0;
