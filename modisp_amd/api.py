@@ -276,6 +276,11 @@ class DispersionBatch:
         self.mode_convention = ("ascending recovered roots; close-cluster omissions may shift indices"
                                 if self.approximate_search.any() else "ascending roots in the reported search interval")
 
+    def check_modes(self, **options):
+        """Cross-period mode checks of this result (modisp_amd.modes)."""
+        from .modes import check_mode_reliability
+        return check_mode_reliability(self, **options)
+
     @property
     def auto_relaxed(self):
         return (self.trace_flags & TRACE_AUTO_RELAXED) != 0 if self.accuracy == "auto" else None
@@ -601,6 +606,10 @@ class Solver:
             continuation_local_success_rate=stats.get("continuation_accepted", 0) / attempts if attempts else None,
             continuity_check=self.continuity_check and _PRESETS[self.accuracy].approximate,
             workspace_budget_bytes=self.workspace_bytes)
+        if self.accuracy == "auto":
+            stats["auto_relaxed_traces"] = int(np.count_nonzero(flags & TRACE_AUTO_RELAXED))
+            stats["sampling_rtol_max"] = (None if self.search_strategy != "shape_adaptive" else
+                                          .05 if self._auto_sampling else self.sampling_rtol)
         if self.profile:
             stats["timings_s"] = {"solver_total": time.perf_counter() - clock}
         self._last_stats = stats
