@@ -24,7 +24,10 @@ solve + misfit loss + adjoint backward on the differentiation workload
 torch.autograd binding.
 
 `uneven` times the ragged uneven batch (65536 models, per-model rows,
-periods and modes) with the independent search and with anchor continuation.
+periods and modes) resident on the device: eagerly with the default launch
+shape, eagerly with the autotuned shape, and replayed from one captured
+hipGraph (`value` of the leg); anchor continuation is timed beside it with its
+acceptance, fallback reasons and FP64 evaluations per target.
 
 `sweep` is the inversion-scale batch (262144 models x 31 rows, 200 periods,
 Rayleigh and Love, 4 modes, group velocity): forward roots/s, forward +
@@ -34,7 +37,11 @@ adjoint gradient models/s and the device memory in use.
 algorithmic FLOP (secular evaluations x layer steps x 160 FLOP per Dunkin
 layer step, counted on the device) divided by its HIP-event time on the
 solver's stream. `cpu_baseline` is the oracle (C++17 FP64 restatement of the
-reference CPU backend) on a bounded sample of the same workload.
+reference CPU backend) on the host's physical cores (capped by the job's CPU
+share): a 32-model sample of the same workload per call, timed by the
+reference's backend protocol (1 first + 3 warm-up + 12 steady calls, median
+and quartiles), and the plain FP64 configuration (1024 models x 7 rows) in
+full beside the device's plain-FP64 and FP32-scan paths on it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -126,23 +133,79 @@ def timed_steps(torch, cases, steps, warmup):
     return wall, [s / steps for s in stage]
 
 
-def cpu_baseline(wl, window, gpu_cases, threads, sample):
-    """The oracle on the first `sample` models of the workload (both waves, all
-    periods, analytic group velocity), timed on the host; also checks the
-    device roots of those models against it."""
+def cpu_threads():
+    """Threads for the CPU baseline: the physical cores among the CPUs this
+    process may run on, capped by the job's CPU share where the launcher
+    states one (OMP_NUM_THREADS). Returns (threads, physical, logical, share)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            cores.add((Path(base + "physical_package_id").read_text().strip(),
+                       Path(base + "core_id").read_text().strip()))
+        except OSError:
+            cores.add(("cpu", c))
+    share = os.environ.get("OMP_NUM_THREADS")
+    share = int(share) if share and share.isdigit() and int(share) > 0 else None
+    threads = min(len(cores), share) if share else len(cores)
+    return max(1, threads), len(cores), len(cpus), share
+
+
+def timing_protocol(call, warmups=3, rounds=12):
+    """One first call, `warmups` warm-up calls and `rounds` steady calls of
+    call() -> roots; median and quartiles of the steady calls (the
+    reference's backend-benchmark protocol, docs/BACKEND_BENCHMARK_0_4_0.md:19)."""
+    import numpy as np
+    samples = {"first": None, "warmup": [], "steady": []}
+    roots = 0
+    for phase, n in (("first", 1), ("warmup", warmups), ("steady", rounds)):
+        for _ in range(n):
+            t0 = time.perf_counter()
+            roots = call()
+            dt = time.perf_counter() - t0
+            if phase == "first":
+                samples["first"] = dt
+            else:
+                samples[phase].append(dt)
+    t = np.array(samples["steady"])
+    med = float(np.median(t))
+    return {"roots_per_call": int(roots), "median_s": med, "q25_s": float(np.quantile(t, .25)),
+            "q75_s": float(np.quantile(t, .75)), "value": roots / med, "samples_s": samples}
+
+
+def cpu_baseline(wl, window, gpu_cases, sample, warmups, rounds, plain_gpu, plain_rounds):
+    """The oracle (C++17 FP64 restatement of the reference CPU backend:
+    shape-free qe_adaptive ladder, Brent to 1e-12, Love phase counting,
+    analytic group velocity) on the host cores:
+
+    * precision sample: the first `sample` models of the workload, both waves,
+      all periods, timed by the protocol (1 + warmups + rounds calls, median);
+      the device roots of those models are checked against it;
+    * cfg0 in full: the plain FP64 batch (1024 models x 7 rows, 60 periods,
+      Rayleigh, 3 modes, phase only), 1 + 0 + plain_rounds calls, beside the
+      device plain-FP64 and FP32-scan paths on the same batch."""
     import numpy as np
     import oracle
     oracle.build(force=False)
+    threads, physical, logical, share = cpu_threads()
     sub = wl.subset(np.arange(sample))
-    roots, elapsed = 0, 0.0
+    refs = {}
+
+    def precision_call():
+        n = 0
+        for case in gpu_cases:
+            ref = oracle.solve(sub.thickness, sub.vp, sub.vs, sub.rho, sub.periods, window[:sample], case.M,
+                               case.wave, tol=1e-12, group=2, threads=threads)
+            refs[case.wave] = ref
+            n += int(np.isfinite(ref["phase_velocity"]).sum())
+        return n
+
+    prec = timing_protocol(precision_call, warmups, rounds)
     agree = {"count_match": 1.0, "mask_match": 1.0, "max_rel_phase": 0.0, "max_rel_group": 0.0}
     for case in gpu_cases:
-        t0 = time.perf_counter()
-        ref = oracle.solve(sub.thickness, sub.vp, sub.vs, sub.rho, sub.periods, window[:sample], case.M, case.wave,
-                           tol=1e-12, group=2, threads=threads)
-        elapsed += time.perf_counter() - t0
+        ref = refs[case.wave]
         rv = ref["phase_velocity"]
-        roots += int(np.isfinite(rv).sum())
         B, M, T = case.B, case.M, case.T
         gv = case.velocity.view(B, M, T)[:sample].cpu().numpy()
         gc = case.counts.view(B, T)[:sample].cpu().numpy()
@@ -158,11 +221,53 @@ def cpu_baseline(wl, window, gpu_cases, threads, sample):
         if gb.any():
             agree["max_rel_group"] = max(agree["max_rel_group"],
                                          float(np.max(np.abs(gg[gb] - rg[gb]) / np.abs(rg[gb]))))
-    return {"value": roots / elapsed, "unit": "roots/s", "cores": threads, "kind": "port",
+    pw, pwin = plain_gpu["workload"], plain_gpu["window"]
+    plain_ref = {}
+
+    def plain_call():
+        r = oracle.solve(pw.thickness, pw.vp, pw.vs, pw.rho, pw.periods, pwin, 3, 0, tol=1e-12, group=0,
+                         threads=threads)
+        plain_ref["r"] = r
+        return int(np.isfinite(r["phase_velocity"]).sum())
+
+    plain = timing_protocol(plain_call, 0, plain_rounds)
+    r = plain_ref["r"]
+    gv = plain_gpu["velocity"]
+    plain["gpu_count_match"] = float(np.mean(plain_gpu["counts"] == r["root_count"]))
+    plain["gpu_mask_match"] = float(np.mean(np.isfinite(gv) == np.isfinite(r["phase_velocity"])))
+    both = np.isfinite(gv) & np.isfinite(r["phase_velocity"])
+    plain["gpu_max_rel_phase"] = float(np.max(np.abs(gv[both] - r["phase_velocity"][both]) / r["phase_velocity"][both]))
+    plain["gpu"] = plain_gpu["rates"]
+    where = f"{threads} threads ({physical} physical cores in the affinity mask of {logical} CPUs" + (
+        f", job CPU share {share})" if share else ")")
+    return {"value": prec["value"], "unit": "roots/s", "cores": threads, "kind": "port",
             "sample": f"first {sample} of {wl.B} models of the workload, Rayleigh+Love, {gpu_cases[0].M} modes, "
-                      f"{gpu_cases[0].T} periods, analytic group velocity: {roots} roots in {elapsed:.2f} s "
-                      f"on {threads} threads",
-            "gpu_parity_on_sample": agree}
+                      f"{gpu_cases[0].T} periods, analytic group velocity, {prec['roots_per_call']} roots per call; "
+                      f"median of {rounds} steady calls after 1 first + {warmups} warm-up calls on {where}",
+            "protocol": prec, "physical_cores": physical, "logical_cpus": logical, "cpu_share": share,
+            "gpu_parity_on_sample": agree,
+            "plain_fp64_full": dict(plain, workload="plain: 1024 models x 7 rows, 60 periods 1-50 s, Rayleigh, 3 "
+                                                    "modes, phase only (the CPU-runnable configuration, in full)")}
+
+
+def plain_gpu_leg(torch, modisp_amd, workloads, steps, warmup):
+    """The plain FP64 batch on the device: plain-FP64 and FP32-scan paths
+    (device-resident roots/s), and the FP64 path's results for the CPU check."""
+    wl = workloads.plain_batch()
+    window = workloads.scan_window(wl.vs)
+    rates = {}
+    keep = None
+    for scan in ("float64", "float32"):
+        case = DeviceCase(modisp_amd, torch, wl, window, 0, scan, group=False)
+        wall, _ = timed_steps(torch, [case], steps, warmup)
+        roots = case.valid_roots()
+        rates[scan] = {"value": roots * steps / wall, "unit": "roots/s", "ms_per_step": wall / steps * 1e3,
+                       "valid_roots": roots}
+        if scan == "float64":
+            keep = (case.velocity.view(case.B, case.M, case.T).cpu().numpy(),
+                    case.counts.view(case.B, case.T).cpu().numpy())
+        case.close()
+    return {"workload": wl, "window": window, "velocity": keep[0], "counts": keep[1], "rates": rates}
 
 
 def end_to_end_leg(wl, steps):
@@ -256,40 +361,91 @@ def gradient_leg(torch, modisp_amd, workloads, steps, warmup):
                        "parameters": "h, vp, vs, rho of every row", "dtype": "f32 scan, f64 refine and adjoint"}}
 
 
-def uneven_leg(modisp_amd, workloads, models):
+def uneven_leg(torch, modisp_amd, workloads, models, steps):
     """The uneven batch (65536 models of 4..41 rows, 20..200 periods 0.2-100 s,
-    1..10 Rayleigh modes each) in one ragged call, independent search and anchor
-    continuation: device roots/s (HIP events around the whole call on the
-    solver stream; host packing and PCIe transfers of the ragged arrays are
-    included) and the continuation's acceptance."""
+    1..10 Rayleigh modes each) resident on the device as one ragged batch
+    (md_rocm_run_ragged_device: preparation, scan, rechecks and refinement on
+    the device). Timed eagerly with the default launch shape, then the
+    autotuner picks the launch shape, then the whole sequence is captured in
+    one hipGraph and replayed: HIP-event milliseconds per call (median of
+    `steps`) and wall milliseconds per call for each. Anchor continuation runs
+    through the host ragged entry (its anchors are planned from host periods):
+    device milliseconds and acceptance."""
     import numpy as np
     wl = workloads.uneven_batch(B=models)
-    lay = wl.layers
-    row_off = np.concatenate([[0], np.cumsum(lay)]).astype(np.int64)
-    cols = [np.concatenate([a[i, :lay[i]] for i in range(wl.B)]) for a in (wl.thickness, wl.vp, wl.vs, wl.rho)]
-    per = np.concatenate(wl.periods)
-    per_off = np.concatenate([[0], np.cumsum([p.size for p in wl.periods])]).astype(np.int64)
-    bounds = np.array([[0.8 * wl.vs[i, :lay[i]].min(), np.nextafter(wl.vs[i, lay[i] - 1], 0.)] for i in range(wl.B)])
-    out = {}
-    for cont in (False, True):
-        with modisp_amd.QeSolver(wl.B, int(lay.max()), 1, 1, continuation=cont, scan="float32") as s:
-            s.run_ragged(*cols, row_off, per, per_off, wl.nmodes, bounds, 1e-12, 0)      # warm-up
+    packed = workloads.pack_ragged(wl)
+    geom, cells = modisp_amd.ragged_geometry(packed["row_offsets"], packed["period_offsets"], packed["modes"])
+    dev = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    bufs = {k: t(packed[k]) for k in ("thickness", "vp", "vs", "rho", "row_offsets", "periods", "period_offsets",
+                                      "modes", "bounds")}
+    bufs["cell_offsets"] = t(cells)
+    bufs.update(velocity=torch.empty(geom["ncells"], dtype=torch.float64, device=dev),
+                flags=torch.empty(geom["ntraces"], dtype=torch.int32, device=dev),
+                levels=torch.empty(geom["ntraces"], dtype=torch.int32, device=dev),
+                counts=torch.empty(geom["ntraces"], dtype=torch.int32, device=dev))
+    ptr = {k: v.data_ptr() for k, v in bufs.items()}
+
+    def timed(solver):
+        ev, wall = [], []
+        for _ in range(steps):
             t0 = time.perf_counter()
-            r = s.run_ragged(*cols, row_off, per, per_off, wl.nmodes, bounds, 1e-12, 0)
-            wall = time.perf_counter() - t0
-            dev_ms = float(s.last_timing()[0])
-        roots = int(np.isfinite(r["velocity"]).sum())
+            solver.run_ragged_device_batch(geom, ptr, 1e-12, 0)
+            solver.synchronize()
+            wall.append(time.perf_counter() - t0)
+            ev.append(float(solver.last_timing()[0]))
+        return {"device_ms": float(np.median(ev)), "wall_ms": float(np.median(wall)) * 1e3,
+                "device_ms_samples": ev}
+
+    out = {}
+    B, R = geom["B"], geom["max_rows"]
+    with modisp_amd.QeSolver(B, R, 1, 1, scan="float32") as s:
+        s.run_ragged_device_batch(geom, ptr, 1e-12, 0)
+        out["eager_default"] = timed(s)
+        out["eager_default"]["tiles"] = {"scan_waves": 4, "consumer_blocks": 8}
+        w, b = s.autotune_ragged_device_batch(geom, ptr, 1e-12, 0)
+        out["eager_tuned"] = timed(s)
+        out["eager_tuned"]["tiles"] = {"scan_waves": w, "consumer_blocks": b}
+        ref = bufs["velocity"].cpu().numpy().copy()
+    with modisp_amd.QeSolver(B, R, 1, 1, scan="float32", graph=True, scan_waves=w, consumer_blocks=b) as g:
+        g.run_ragged_device_batch(geom, ptr, 1e-12, 0)      # eager run, then capture
+        g.synchronize()
+        out["graph_replay"] = timed(g)
+        out["graph_replay"]["tiles"] = {"scan_waves": w, "consumer_blocks": b}
+        got = bufs["velocity"].cpu().numpy()
+        out["graph_replay"]["equals_eager"] = bool(np.array_equal(got.view(np.uint64), ref.view(np.uint64)))
+    roots = int(np.isfinite(ref).sum())
+    out["roots"] = roots
+    out["value"] = roots / (out["graph_replay"]["device_ms"] * 1e-3)
+    out["unit"] = "roots/s"
+    out["graph_over_eager"] = out["eager_tuned"]["device_ms"] / out["graph_replay"]["device_ms"]
+    args = [packed[k] for k in ("thickness", "vp", "vs", "rho", "row_offsets", "periods", "period_offsets", "modes",
+                                "bounds")]
+    with modisp_amd.QeSolver(B, R, 1, 1, continuation=True, scan="float32") as s:
+        s.run_ragged(*args, 1e-12, 0)      # warm-up
+        t0 = time.perf_counter()
+        r = s.run_ragged(*args, 1e-12, 0)
+        wall = time.perf_counter() - t0
         pred = r["prediction_flags"]
-        key = "continuation" if cont else "independent"
-        out[key] = {"value": roots / (dev_ms * 1e-3), "unit": "roots/s", "device_ms": dev_ms, "wall_ms": wall * 1e3,
-                    "roots": roots}
-        if cont:
-            targets = int(np.count_nonzero((pred & 1) == 0))
-            out[key]["targets"] = targets
-            out[key]["accepted"] = int(np.count_nonzero(pred & 2))
-            out[key]["fallbacks"] = int(np.count_nonzero(pred & 4))
-    out["config"] = {"workload": f"uneven: {wl.B} models, {int(per_off[-1])} traces, rows 4..41, periods 20..200, "
-                                 "modes 1..10, Rayleigh, 20% low-velocity zones", "entry_point": "md_rocm_run_ragged"}
+        st = s.last_stats()
+        fell = (pred & 4) != 0
+        reasons = {name: int(np.count_nonzero(fell & ((pred & bit) != 0)))
+                   for name, bit in (("source_uncertain", 64), ("root_count", 128), ("anchor_unsafe", 256),
+                                     ("prediction_window", 512), ("bracket", 1024), ("correction", 2048),
+                                     ("audit", 32))}
+        out["continuation"] = {"device_ms": float(s.last_timing()[0]), "wall_ms": wall * 1e3,
+                               "fp64_evaluations": st["continuation_evaluations"],
+                               "fp64_evaluations_per_target": st["continuation_evaluations"] / max(1, st[
+                                   "continuation_targets"]),
+                               "fallback_reasons": reasons,
+                               "roots": int(np.isfinite(r["velocity"]).sum()),
+                               "targets": int(np.count_nonzero((pred & 1) == 0)),
+                               "accepted": int(np.count_nonzero(pred & 2)),
+                               "fallbacks": int(np.count_nonzero(pred & 4)),
+                               "entry_point": "md_rocm_run_ragged (host arrays; PCIe copies inside the events)"}
+    out["config"] = {"workload": f"uneven: {B} models, {geom['ntraces']} traces, rows 4..41, periods 20..200, "
+                                 "modes 1..10, Rayleigh, 20% low-velocity zones",
+                     "entry_point": "md_rocm_run_ragged_device + md_rocm_autotune_ragged_device, hipGraph replay"}
     return out
 
 
@@ -360,14 +516,17 @@ def sweep_leg(torch, modisp_amd, workloads, steps):
                        "steps": steps}}
 
 
-def traffic_per_launch():
-    """HBM bytes per scan launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (scaled by a copy-kernel calibration), if present."""
+def scan_traffic():
+    """HBM bytes of the Rayleigh scan from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by a copy-kernel calibration), if present:
+    (bytes per run -- one pass over the batch, two trace-chunk dispatches --,
+    mean bytes per dispatch)."""
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
-        return float(d["fetch_bytes_per_dispatch"] + d["write_bytes_per_dispatch"])
+        return (float(d["fetch_bytes_per_run"] + d["write_bytes_per_run"]),
+                float(d["fetch_bytes_per_dispatch"] + d["write_bytes_per_dispatch"]))
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
 
 
 def main():
@@ -376,7 +535,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--models", type=int, default=16384)
-    ap.add_argument("--cpu-sample", type=int, default=512, help="models in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=32, help="models per call of the CPU baseline protocol")
+    ap.add_argument("--cpu-plain-rounds", type=int, default=2,
+                    help="steady calls of the CPU baseline on the full plain FP64 batch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uniform", action="store_true", help="skip the 512-sample uniform-grid comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
@@ -443,8 +604,12 @@ def main():
     scan_ms = float(st32[0][1])
     achieved = flop / (scan_ms * 1e-3) / 1e12
     love_flop = stats32[1]["scan_nodes"] * (L - 1) * FLOP_PER_LAYER[1]
+    traffic_run, traffic_dispatch = scan_traffic()
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32, "traffic": traffic_per_launch(),
+                "frac": achieved / PEAK_FP32, "traffic": traffic_run,
+                "traffic_scope": "HBM bytes per run of the kernel over the batch (the same scope as "
+                                 "algorithmic_flop_per_run and kernel_ms_per_run)",
+                "traffic_per_dispatch": traffic_dispatch,
                 "kernel": "scan_kernel<float,64,4> (Rayleigh)",
                 "algorithmic_flop_per_run": flop, "secular_evaluations_per_run": evals,
                 "flop_per_layer_step": FLOP_PER_LAYER[0], "layer_steps_per_evaluation": L - 1,
@@ -489,9 +654,10 @@ def main():
         log(f"fwd+adjoint: {result['fwd_adjoint']['ms_per_step']:.2f} ms/step, "
             f"{result['fwd_adjoint']['value']:.0f} models/s")
     if not args.no_uneven:
-        result["uneven"] = uneven_leg(modisp_amd, workloads, args.uneven_models)
+        result["uneven"] = uneven_leg(torch, modisp_amd, workloads, args.uneven_models, args.steps)
         u = result["uneven"]
-        log(f"uneven: independent {u['independent']['device_ms']:.1f} ms, continuation "
+        log(f"uneven: eager {u['eager_default']['device_ms']:.1f} ms, tuned {u['eager_tuned']['device_ms']:.1f} ms "
+            f"{u['eager_tuned']['tiles']}, graph replay {u['graph_replay']['device_ms']:.1f} ms; continuation "
             f"{u['continuation']['device_ms']:.1f} ms ({u['continuation']['accepted']} of "
             f"{u['continuation']['targets']} targets accepted)")
     if not args.no_sweep:
@@ -501,8 +667,12 @@ def main():
             f"{sw['fwd_adjoint']['ms_per_step']:.1f} ms/step, {sw['fwd_adjoint']['value']:.0f} models/s; "
             f"HBM in use {sw['hbm_in_use_bytes']['forward_adjoint'] / 2**30:.1f} GiB")
     if not args.no_cpu:
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
-        result["cpu_baseline"] = cpu_baseline(wl, window, fast, threads, min(args.cpu_sample, wl.B))
+        plain_gpu = plain_gpu_leg(torch, modisp_amd, workloads, args.steps, args.warmup)
+        result["cpu_baseline"] = cpu_baseline(wl, window, fast, min(args.cpu_sample, wl.B), 3, 12, plain_gpu,
+                                              args.cpu_plain_rounds)
+        cb = result["cpu_baseline"]
+        log(f"cpu baseline: {cb['value']:.3g} roots/s on {cb['cores']} threads (median of 12); plain FP64 full "
+            f"{cb['plain_fp64_full']['value']:.3g} roots/s")
     for c in fast:
         c.close()
     print(json.dumps(result))

@@ -55,8 +55,9 @@ typedef struct md_rocm_config {
   int chunk_traces;        /* traces per device chunk (0 = automatic) */
   int grid;                /* 1: density-adaptive nodes (default), 0: uniform nc-node grid */
   double rho;              /* adaptive grid: nodes per WKB mode (default 8) */
-  int graph;               /* 1: md_rocm_run_device captures its launch sequence in a hipGraph
-                              and replays it for repeated calls with identical arguments */
+  int graph;               /* 1: md_rocm_run_device and md_rocm_run_ragged_device capture their
+                              launch sequence in a hipGraph and replay it for repeated calls
+                              with identical arguments */
   int scan_waves;          /* traces (wavefronts) per scan block: 1, 2, 4 or 8 (FP64: <= 4);
                               0 = 4, or what md_rocm_autotune picked */
   int consumer_blocks;     /* persistent consumer blocks per compute unit, 1..64 (0 = 8) */
@@ -192,6 +193,43 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
                        double c_tol_rel, int wave, double *velocity, uint32_t *flags, int32_t *levels,
                        int32_t *counts, double *group, uint32_t *group_flags, uint32_t *prediction_flags);
 
+/* Ragged batch resident in DEVICE memory (HBM), the device counterpart of
+ * md_rocm_run_ragged: every pointer is a device address. row_offsets,
+ * period_offsets and cell_offsets have B + 1 entries (model m owns rows
+ * [row_offsets[m], row_offsets[m+1]) of the packed thickness/vp/vs/rho arrays
+ * of nrows rows, traces [period_offsets[m], period_offsets[m+1]) of the packed
+ * periods and of flags/levels/counts (ntraces entries), and velocity/group cells
+ * [cell_offsets[m], cell_offsets[m+1]) with cell = cell_offsets[m] + mode * T_m + f
+ * (ncells entries)); modes[m] reported modes; bounds [B, 2]. max_rows and
+ * max_modes bound every model's rows and modes (they size the kernels). The
+ * preparation runs on the device; a model whose offsets are out of range or
+ * inconsistent (cell range != modes x periods) is solved as an empty model
+ * and counted (md_rocm_last_stats value 9), and no kernel touches memory
+ * outside the given extents. With cfg.graph the whole sequence (preparation,
+ * scan, rechecks, refinement, group velocity) is captured once and replayed
+ * for calls with identical arguments. Asynchronous on the solver's stream.
+ * Anchor continuation (cfg.continuation) plans from host periods and is
+ * refused here. No reference counterpart (the reference requires rectangular
+ * batches, python/modisp/models.py:224-270, and keeps them on the host). */
+int md_rocm_run_ragged_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                              const double *rho, const int64_t *row_offsets, long long nrows, int B,
+                              const double *periods, const int64_t *period_offsets, long long ntraces,
+                              const int32_t *modes, const int64_t *cell_offsets, long long ncells,
+                              const double *bounds, int max_rows, int max_modes, double c_tol_rel, int wave,
+                              double *velocity, uint32_t *flags, int32_t *levels, int32_t *counts, double *group,
+                              uint32_t *group_flags);
+
+/* md_rocm_autotune for a device ragged batch (arguments as
+ * md_rocm_run_ragged_device; outputs are overwritten). No reference
+ * counterpart. */
+int md_rocm_autotune_ragged_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                                   const double *rho, const int64_t *row_offsets, long long nrows, int B,
+                                   const double *periods, const int64_t *period_offsets, long long ntraces,
+                                   const int32_t *modes, const int64_t *cell_offsets, long long ncells,
+                                   const double *bounds, int max_rows, int max_modes, double c_tol_rel, int wave,
+                                   double *velocity, uint32_t *flags, int32_t *levels, int32_t *counts,
+                                   double *group, uint32_t *group_flags, int *scan_waves, int *consumer_blocks);
+
 /* Wait for the solver's stream; returns the first asynchronous error. */
 int md_rocm_synchronize(md_rocm_solver *s);
 
@@ -207,8 +245,12 @@ int md_rocm_last_timing(md_rocm_solver *s, double *ms, int n);
  * 2 hidden-pair probes, 3 FP64 rescans of overflowed FP32 traces, 4 Love phase
  * searches, 5 secular evaluations of the grid scans, 6 FP64 evaluations of
  * the bracket refinement, 7 FP32 and 8 FP64 evaluations of the hidden-pair
- * probes. Synchronizes the solver's stream; returns the number of values
- * written (at most 9) or -1 on error.
+ * probes, 9 models of the last md_rocm_run_ragged_device call with
+ * inconsistent offsets (0 after other entry points), and for the last run
+ * with anchor continuation: 10 continuation targets, 11 targets accepted
+ * without the full pipeline, 12 FP64 evaluations (secular or Love phase) of
+ * the continuation kernel. Synchronizes the solver's stream; returns the
+ * number of values written (at most 13) or -1 on error.
  * No reference counterpart: the reference returns these only from run()
  * (cpp/cuda/qe.cu:815-821). */
 int md_rocm_last_stats(md_rocm_solver *s, uint64_t *stats, int n);

@@ -33,9 +33,10 @@ SUPPORTS_ROW_PERIODS = True
 SUPPORTS_GROUP_VELOCITY = True
 SUPPORTS_CONTINUATION = True
 STAT_NAMES = ("traces", "fp64_rechecks", "pair_probes", "fp64_rescans", "love_phase_searches", "scan_nodes",
-              "refine_evaluations", "probe_evaluations", "probe_evaluations_fp64")
+              "refine_evaluations", "probe_evaluations", "probe_evaluations_fp64", "invalid_models",
+              "continuation_targets", "continuation_accepted", "continuation_evaluations")
 
-__all__ = ["load", "QeSolver", "available", "set_device", "memory_info", "device_name", "bounds",
+__all__ = ["load", "QeSolver", "ragged_geometry", "available", "set_device", "memory_info", "device_name", "bounds",
            "BackendUnavailableError", "ABI_VERSION", "VERSION"]
 
 
@@ -220,6 +221,26 @@ def probe_grid(thickness, vp, vs, rho, periods, bounds, nc=512, wave=0, fp32=Tru
     return out
 
 
+def ragged_geometry(row_offsets, period_offsets, modes):
+    """Checks the host offsets of a ragged batch and returns (geometry,
+    cell_offsets): geometry holds B, nrows, ntraces, ncells, max_rows and
+    max_modes for run_ragged_device_batch, cell_offsets [B+1] (int64) the
+    packed velocity cells of each model (modes[m] x periods of m)."""
+    ro = np.asarray(row_offsets, dtype=np.int64)
+    po = np.asarray(period_offsets, dtype=np.int64)
+    md = np.asarray(modes, dtype=np.int64)
+    B = md.size
+    if B < 1 or ro.shape != (B + 1,) or po.shape != (B + 1,) or ro[0] != 0 or po[0] != 0:
+        raise ValueError("row_offsets and period_offsets need B + 1 entries starting at 0")
+    rows, T = np.diff(ro), np.diff(po)
+    if rows.min() < 1 or T.min() < 0 or md.min() < 1 or md.max() > 256:
+        raise ValueError("every model needs >= 1 row, >= 0 periods and 1..256 modes")
+    cell = np.concatenate([[0], np.cumsum(md * T)]).astype(np.int64)
+    geometry = dict(B=int(B), nrows=int(ro[-1]), ntraces=int(po[-1]), ncells=int(cell[-1]), max_rows=int(rows.max()),
+                    max_modes=int(md.max()))
+    return geometry, cell
+
+
 class QeSolver:
     """Persistent native engine; the constructor and run() mirror the reference
     QeSolver (cpp/cuda/qe.cu:1166 signature, run at qe.cu:1060-1118).
@@ -375,6 +396,37 @@ class QeSolver:
         _native.check(code)
         return dict(velocity=vel, flags=flags, levels=levels, counts=counts, group_velocity=gv,
                     group_velocity_flags=gf, prediction_flags=pred)
+
+    _RAGGED_KEYS = ("thickness", "vp", "vs", "rho", "row_offsets", "periods", "period_offsets", "modes",
+                    "cell_offsets", "bounds", "velocity", "flags", "levels", "counts")
+
+    def _ragged_args(self, geometry, pointers, c_tol_rel, wave):
+        g, p = geometry, pointers
+        missing = [k for k in self._RAGGED_KEYS if not p.get(k)]
+        if missing:
+            raise ValueError(f"device pointers missing: {missing}")
+        return (self.handle, p["thickness"], p["vp"], p["vs"], p["rho"], p["row_offsets"], int(g["nrows"]), int(g["B"]),
+                p["periods"], p["period_offsets"], int(g["ntraces"]), p["modes"], p["cell_offsets"], int(g["ncells"]),
+                p["bounds"], int(g["max_rows"]), int(g["max_modes"]), float(c_tol_rel), int(wave), p["velocity"],
+                p["flags"], p["levels"], p["counts"], p.get("group"), p.get("group_flags"))
+
+    def run_ragged_device_batch(self, geometry, pointers, c_tol_rel, wave):
+        """Ragged batch resident on the device (md_rocm_run_ragged_device).
+        geometry: B, nrows, ntraces, ncells, max_rows, max_modes (see
+        ragged_geometry); pointers: device addresses of thickness, vp, vs, rho
+        [nrows], row_offsets, period_offsets, cell_offsets [B+1] (int64), modes
+        [B] (int32), periods [ntraces], bounds [B, 2], velocity [ncells],
+        flags, levels, counts [ntraces][, group, group_flags [ncells]].
+        With graph=True repeated calls replay one captured hipGraph."""
+        _native.check(_native.lib().md_rocm_run_ragged_device(*self._ragged_args(geometry, pointers, c_tol_rel, wave)))
+
+    def autotune_ragged_device_batch(self, geometry, pointers, c_tol_rel, wave):
+        """autotune_device_batch for a device ragged batch; returns (traces per
+        scan block, consumer blocks per CU)."""
+        w, b = ctypes.c_int(0), ctypes.c_int(0)
+        _native.check(_native.lib().md_rocm_autotune_ragged_device(
+            *self._ragged_args(geometry, pointers, c_tol_rel, wave), ctypes.byref(w), ctypes.byref(b)))
+        return w.value, b.value
 
     def run_device_batch(self, B, pointers, c_tol_rel, wave, row_periods=False):
         """pointers: dict of device addresses (int) for thickness, vp, vs, rho,

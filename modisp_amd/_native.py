@@ -46,7 +46,7 @@ _lib = None
 
 # Every symbol declared in include/modisp_rocm.h.
 EXPORTS = ("md_rocm_default_config", "md_rocm_create", "md_rocm_free", "md_rocm_run", "md_rocm_run_device",
-           "md_rocm_run_ragged", "md_rocm_synchronize", "md_rocm_last_timing", "md_rocm_bounds",
+           "md_rocm_run_ragged", "md_rocm_run_ragged_device", "md_rocm_autotune_ragged_device", "md_rocm_synchronize", "md_rocm_last_timing", "md_rocm_bounds",
            "md_rocm_available", "md_rocm_set_device", "md_rocm_memory_info", "md_rocm_device_name",
            "md_rocm_last_error", "md_rocm_probe_grid", "md_rocm_last_stats", "md_rocm_adjoint",
            "md_rocm_set_stream", "md_rocm_autotune", "md_rocm_qe_kernel_report",
@@ -95,6 +95,14 @@ def lib():
     L.md_rocm_run_ragged.argtypes = [_vp, _dp, _dp, _dp, _dp, _i64p, ctypes.c_int, _dp, _i64p, _i32p, _dp,
                                      ctypes.c_double, ctypes.c_int, _dp, _u32p, _i32p, _i32p, _dp, _u32p, _u32p]
     L.md_rocm_run_ragged.restype = ctypes.c_int
+    ll = ctypes.c_longlong
+    L.md_rocm_run_ragged_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ll, ctypes.c_int, _vp, _vp, ll, _vp, _vp,
+                                            ll, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp,
+                                            _vp, _vp, _vp, _vp, _vp]
+    L.md_rocm_run_ragged_device.restype = ctypes.c_int
+    L.md_rocm_autotune_ragged_device.argtypes = L.md_rocm_run_ragged_device.argtypes + [
+        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.md_rocm_autotune_ragged_device.restype = ctypes.c_int
     L.md_rocm_synchronize.argtypes = [_vp]
     L.md_rocm_synchronize.restype = ctypes.c_int
     L.md_rocm_last_timing.argtypes = [_vp, _dp, ctypes.c_int]

@@ -1599,12 +1599,14 @@ __device__ unsigned cont_predict(const double *cl0, const double *cr0, const dou
 // Golden-section search for a point of the opposite sign between two
 // same-sign samples (an even root pair); true if one is found.
 __device__ bool cont_hidden_pair(const PrepRow *rows, int L, int top, int wave, double omega, double lo, double hi,
-                                 double sign) {
+                                 double sign, int &ev) {
   constexpr double phi = 0.6180339887498949;
   double p = hi - phi * (hi - lo), r = lo + phi * (hi - lo);
   double fp = sign * secular_rows(rows, L, top, wave, omega, p), fr = sign * secular_rows(rows, L, top, wave, omega, r);
+  ev += 2;
   for (int it = 0; it < 40; ++it) {
     if (fp <= 0. || fr <= 0.) return true;
+    ++ev;
     if (fp < fr) { hi = r; r = p; fr = fp; p = hi - phi * (hi - lo); fp = sign * secular_rows(rows, L, top, wave, omega, p); }
     else { lo = p; p = r; fp = fr; r = lo + phi * (hi - lo); fr = sign * secular_rows(rows, L, top, wave, omega, r); }
   }
@@ -1615,9 +1617,10 @@ __device__ bool cont_hidden_pair(const PrepRow *rows, int L, int top, int wave, 
 // phase levels at the bracket ends; Rayleigh by three samples on each open
 // side of the root, with a pair search at a same-sign trough.
 __device__ unsigned cont_window(const PrepRow *rows, int L, int top, int wave, double omega, double a, double b,
-                                double fa, double fb, double root, double tol, double level, double vs_half) {
+                                double fa, double fb, double root, double tol, double level, double vs_half, int &ev) {
   if (wave) {
     constexpr double pi = 3.141592653589793;
+    ev += 2;
     const double left = love_phase_dev(rows, L, top, omega, a, vs_half),
                  right = love_phase_dev(rows, L, top, omega, b, vs_half);
     if (!(left == left) || !(right == right)) return kPredAuditFailed;
@@ -1631,6 +1634,7 @@ __device__ unsigned cont_window(const PrepRow *rows, int L, int top, int wave, d
     const double end = root + copysign(margin, edge - root), step = (end - edge) * .25;
     double mag[3];
     for (int k = 0; k < 3; ++k) {
+      ++ev;
       const double f = sign * secular_rows(rows, L, top, wave, omega, edge + (k + 1) * step);
       if (!(f == f)) return kPredAuditFailed;
       if (f <= 0.) return kPredAuditFailed | kPredReview;
@@ -1638,7 +1642,8 @@ __device__ unsigned cont_window(const PrepRow *rows, int L, int top, int wave, d
     }
     if (mag[1] < mag[0] && mag[1] < mag[2] && (mag[1] < .5 * fmin(mag[0], mag[2]) || mag[1] < .05)) {
       const double x = edge + step, y = edge + 3. * step;
-      if (cont_hidden_pair(rows, L, top, wave, omega, fmin(x, y), fmax(x, y), sign)) return kPredAuditFailed | kPredReview;
+      if (cont_hidden_pair(rows, L, top, wave, omega, fmin(x, y), fmax(x, y), sign, ev))
+        return kPredAuditFailed | kPredReview;
     }
   }
   return 0u;
@@ -1649,7 +1654,7 @@ __device__ unsigned cont_window(const PrepRow *rows, int L, int top, int wave, d
 // mode count (at least 4 per gap) and at every layer speed inside the gap,
 // none of opposite sign and none a trough that hides a pair.
 __device__ unsigned cont_gaps(const PrepRow *rows, int L, int top, int wave, double omega, double lower,
-                              double upper, const double *lo, const double *hi, int nr, int K) {
+                              double upper, const double *lo, const double *hi, int nr, int K, int &ev) {
   if (nr < 1 || nr > K) return kPredAuditFailed;
   if (nr == K) upper = hi[nr - 1];
   auto density = [&](double c) {
@@ -1666,10 +1671,12 @@ __device__ unsigned cont_gaps(const PrepRow *rows, int L, int top, int wave, dou
     const double a = g ? hi[g - 1] : lower, b = g < nr ? lo[g] : upper;
     if (!(b > a)) continue;
     const double fa = secular_rows(rows, L, top, wave, omega, a), fb = secular_rows(rows, L, top, wave, omega, b);
+    ev += 2;
     if (!(fa == fa) || !(fb == fb)) return kPredAuditFailed;
     if (fa == 0. || fb == 0. || signbit(fa) != signbit(fb)) return kPredAuditFailed | kPredReview;
     const double sign = fa < 0. ? -1. : 1., endpoint = fmin(fabs(fa), fabs(fb));
     auto sample = [&](double c) -> unsigned {
+      ++ev;
       const double f = secular_rows(rows, L, top, wave, omega, c);
       if (!(f == f)) return kPredAuditFailed;
       if (f == 0. || sign * f < 0. || (fabs(f) < .05 && fabs(f) < .1 * endpoint)) return kPredAuditFailed | kPredReview;
@@ -1693,6 +1700,7 @@ __device__ unsigned cont_gaps(const PrepRow *rows, int L, int top, int wave, dou
 
 // One lane per target trace.
 __global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
+  int ev = 0;  // FP64 secular (or Love phase) evaluations of this lane
   for (int q = blockIdx.x * kBlock + threadIdx.x; q < a.ntargets; q += gridDim.x * kBlock) {
     const TargetPlan tp = a.targets[q];
     const TraceInfo ti = a.traces[tp.trace];
@@ -1721,6 +1729,7 @@ __global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
       double first_level = 0.;
       if (a.wave) {
         const double ph = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.lo, mi.vs_half);
+        ++ev;
         if (!(ph == ph)) reason = kPredAuditFailed;
         first_level = floor(ph / 3.141592653589793);
       }
@@ -1734,22 +1743,23 @@ __global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
           x1 = fmin(mi.hi, c + width);
           f0 = secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, x0);
           f1 = secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, x1);
+          ev += 2;
           if (f0 == f0 && f1 == f1 && f0 != 0. && f1 != 0. && signbit(f0) != signbit(f1)) { bracket = true; break; }
           if (width >= cap) break;
           width = fmin(cap, 2. * width);
         }
         if (!bracket) { reason = kPredBracket; break; }
-        int ev = 0;
         const double root = refine_serial(a.refinement, rows, mi.L, mi.top, a.wave, ti.omega, x0, x1, f0, f1, a.tol,
                                           ev);
         if (!(root >= x0 && root <= x1)) { reason = kPredCorrection; break; }
         if (j && root - roots[j - 1] <= a.close_rtol * fabs(root)) { reason = kPredCorrection | kPredReview; break; }
         if (a.f_accept < 1.) {
           const double fr = secular_rows(rows, mi.L, mi.top, a.wave, ti.omega, root);
+          ++ev;
           if (!(fabs(fr) <= a.f_accept)) { reason = kPredCorrection; break; }
         }
         const unsigned issue = cont_window(rows, mi.L, mi.top, a.wave, ti.omega, x0, x1, f0, f1, root, a.tol,
-                                           first_level - j, mi.vs_half);
+                                           first_level - j, mi.vs_half, ev);
         if (issue) { reason = issue; pred |= kPredAudit; break; }
         lo[j] = x0;
         hi[j] = x1;
@@ -1757,7 +1767,7 @@ __global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
       }
       if (!reason) {
         pred |= kPredAudit;
-        reason = cont_gaps(rows, mi.L, mi.top, a.wave, ti.omega, mi.lo, mi.hi, lo, hi, nr, K);
+        reason = cont_gaps(rows, mi.L, mi.top, a.wave, ti.omega, mi.lo, mi.hi, lo, hi, nr, K, ev);
       }
     }
     if (reason) {
@@ -1783,6 +1793,11 @@ __global__ __launch_bounds__(kBlock) void continuation_kernel(ContArgs a) {
     a.flags[ct] = word;
     a.levels[ct] = kLevelContinuation;
     a.counts[ct] = nr;
+  }
+  if (a.evaluations) {
+    unsigned long long e = static_cast<unsigned long long>(ev);
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
+    if ((threadIdx.x & 63) == 0 && e) atomicAdd(a.evaluations, e);
   }
 }
 
@@ -2013,6 +2028,80 @@ hipError_t launch_prep(const double *h, const double *vp, const double *vs, cons
   const long long n = static_cast<long long>(B) * T;
   hipLaunchKernelGGL(prep_traces_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, periods,
                      row_periods, B, T, models, p, traces);
+  return hipGetLastError();
+}
+
+// Ragged batches resident in HBM. Every trace first becomes a dead trace of
+// the sentinel model (index B: empty window, no reported modes, its trace
+// output at the trace's own index); then every model whose offsets are
+// consistent writes its rows, its ModelInfo and its own traces, one wavefront
+// per model. A model whose row, period or cell range is out of bounds, or
+// whose cell range does not hold modes x periods cells, keeps the sentinel
+// and is counted in *invalid, so no later kernel reads or writes outside the
+// caller's arrays, whatever the offsets say.
+__device__ ModelInfo sentinel_model() {
+  ModelInfo mi{};
+  mi.L = 1;
+  mi.K = 2;
+  mi.empty = 1;
+  return mi;
+}
+
+__global__ void prep_dead_traces_kernel(long long ntraces, int sentinel, TraceInfo *traces, ModelInfo *models,
+                                        int *invalid) {
+  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    models[sentinel] = sentinel_model();
+    *invalid = 0;
+  }
+  if (t >= ntraces) return;
+  TraceInfo ti{};
+  ti.omega = 1.;
+  ti.model = sentinel;
+  ti.f = static_cast<int>(t);
+  traces[t] = ti;
+}
+
+__global__ __launch_bounds__(kBlock) void prep_ragged_kernel(const double *h, const double *vp, const double *vs,
+                                                             const double *rho, const double *bounds,
+                                                             const double *periods, RaggedGeometry g, PrepParams p,
+                                                             PrepRow *rows, ModelInfo *models, TraceInfo *traces,
+                                                             int *invalid) {
+  const int lane = threadIdx.x & 63;
+  const int m = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (m >= g.B) return;
+  const long long r0 = g.row_offsets[m], r1 = g.row_offsets[m + 1];
+  const long long p0 = g.period_offsets[m], p1 = g.period_offsets[m + 1];
+  const long long o0 = g.cell_offsets[m], o1 = g.cell_offsets[m + 1];
+  const long long K = g.modes[m], L = r1 - r0, T = p1 - p0;
+  const bool ok = r0 >= 0 && L >= 1 && L <= g.max_rows && r1 <= g.nrows && p0 >= 0 && T >= 0 && p1 <= g.ntraces &&
+                  K >= 1 && K <= g.max_modes && o0 >= 0 && o1 <= g.ncells && o1 - o0 == K * T;
+  if (!ok) {
+    if (lane == 0) {
+      models[m] = sentinel_model();
+      atomicAdd(invalid, 1);
+    }
+    return;
+  }
+  for (long long j = r0 + lane; j < r1; j += 64) rows[j] = make_prep_row(h[j], vp[j], vs[j], rho[j]);
+  double slow;
+  const ModelInfo mi = make_model_info(h + r0, vp + r0, vs + r0, static_cast<int>(L), bounds[2 * m],
+                                       bounds[2 * m + 1], p.wave, static_cast<int>(K), static_cast<int>(T), r0, o0,
+                                       p0, &slow);
+  if (lane == 0) models[m] = mi;
+  for (int f = lane; f < T; f += 64) traces[p0 + f] = make_trace(m, f, periods[p0 + f], slow, p);
+}
+
+hipError_t launch_prep_ragged(const double *h, const double *vp, const double *vs, const double *rho,
+                              const double *bounds, const double *periods, const RaggedGeometry &g,
+                              const PrepParams &p, PrepRow *rows, ModelInfo *models, TraceInfo *traces, int *invalid,
+                              hipStream_t s) {
+  const long long n = g.ntraces > 0 ? g.ntraces : 1;
+  hipLaunchKernelGGL(prep_dead_traces_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
+                     g.ntraces, g.B, traces, models, invalid);
+  const int per_block = kBlock / 64;
+  hipLaunchKernelGGL(prep_ragged_kernel, dim3((g.B + per_block - 1) / per_block), dim3(kBlock), 0, s, h, vp, vs,
+                     rho, bounds, periods, g, p, rows, models, traces, invalid);
   return hipGetLastError();
 }
 

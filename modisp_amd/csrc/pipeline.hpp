@@ -226,6 +226,7 @@ struct ContArgs {
   uint32_t *prediction;        // [trace] (cell = model trace_out0 + f)
   int *fallback;               // trace indices to solve with the full pipeline
   int *fallback_count;
+  unsigned long long *evaluations;  // optional: FP64 evaluations of the continuation kernel (added)
 };
 hipError_t launch_continuation(const ContArgs &c, int persistent_blocks, hipStream_t s);
 hipError_t launch_review_groups(const PrepRow *rows, int L, int top, int wave, double omega, const double *roots,
@@ -333,6 +334,20 @@ namespace mdr {
 hipError_t launch_prep(const double *h, const double *vp, const double *vs, const double *rho, const double *bounds,
                        const double *periods, int row_periods, int B, int L, int T, int modes, const PrepParams &p,
                        PrepRow *rows, ModelInfo *models, TraceInfo *traces, hipStream_t s);
+// Offsets of a ragged batch resident in HBM (device arrays of B + 1 entries)
+// and the extents of the caller's arrays they index.
+struct RaggedGeometry {
+  const int64_t *row_offsets, *period_offsets, *cell_offsets;
+  const int32_t *modes;
+  int B, max_rows, max_modes;
+  long long nrows, ntraces, ncells;
+};
+// Prepares B models and g.ntraces traces (plus the sentinel model B) on the
+// device; models with inconsistent offsets are counted in *invalid.
+hipError_t launch_prep_ragged(const double *h, const double *vp, const double *vs, const double *rho,
+                              const double *bounds, const double *periods, const RaggedGeometry &g,
+                              const PrepParams &p, PrepRow *rows, ModelInfo *models, TraceInfo *traces, int *invalid,
+                              hipStream_t s);
 }  // namespace mdr
 
 namespace mdr {

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -106,7 +107,7 @@ struct md_rocm_solver {
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   DevBuf rows, models, traces, events, scan, brackets, roots, unc_q, tr_q, counters, stats, overflow, phase, tape;
   DevBuf c_anchors, c_targets, c_list, c_roots, c_slopes, c_fallback, c_count, c_pred;
-  DevBuf in_h, in_vp, in_vs, in_rho, in_per, in_bounds;
+  DevBuf in_h, in_vp, in_vs, in_rho, in_per, in_bounds, in_offsets;
   DevBuf out_v, out_flags, out_levels, out_counts, out_group, out_gflags, out_v32;
   double last_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<hipEvent_t> marks;  // kStageMarks per chunk of the last run
@@ -115,19 +116,22 @@ struct md_rocm_solver {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<uintptr_t> graph_key;  // arguments the captured graph was built for
-  long long cont_accepted = 0, cont_fallbacks = 0;  // continuation targets of the last run
+  long long cont_targets = 0, cont_accepted = 0, cont_fallbacks = 0;  // continuation targets of the last run
+  long long cont_evaluations = 0;  // FP64 evaluations of the continuation kernel in the last run
   double f_accept = 1.;            // acceptance threshold of the current call (1 = off)
   bool timed = false;              // ev_begin / ev_end were recorded by the last run
   int prep_models = 0;             // models prepared on the device for the current run
   int prep_rows = 0;               // largest row count among them
   DevBuf table, ntab;              // per-model scan tables (adaptive grid)
+  DevBuf invalid;                  // models of the last device ragged run with inconsistent offsets
+  bool invalid_live = false;       // `invalid` belongs to the last run
 
   ~md_rocm_solver() {
     for (DevBuf *b : {&rows, &models, &traces, &events, &scan, &brackets, &roots, &unc_q, &tr_q, &counters, &stats,
                       &overflow, &phase, &tape, &c_anchors, &c_targets, &c_list, &c_roots, &c_slopes,
                       &c_fallback, &c_count, &c_pred,
-                      &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &out_v, &out_flags, &out_levels,
-                      &out_counts, &out_group, &out_gflags, &out_v32, &table, &ntab})
+                      &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &in_offsets, &out_v, &out_flags, &out_levels,
+                      &out_counts, &out_group, &out_gflags, &out_v32, &table, &ntab, &invalid})
       b->release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -242,6 +246,7 @@ int run_pipeline(md_rocm_solver *s, const Work &w, long long ntraces, int kmax, 
                  uint32_t *gflags, const mdr::TraceInfo *list = nullptr, double *all_roots = nullptr) {
   const int chunk = w.chunk;
   const size_t qcap = static_cast<size_t>(chunk) * mdr::kEventCap;
+  s->invalid_live = false;
   // Per-model scan tables for the adaptive grid (one wavefront per model).
   const int stride = static_cast<int>(table_stride(s->prep_rows));
   if (s->cfg.grid)
@@ -410,11 +415,11 @@ int run_continuation(md_rocm_solver *s, int kmax, int max_rows, int wave, double
   if (int e = s->c_roots.ensure(std::max<size_t>(1, na) * kmax * sizeof(double))) return e;
   if (int e = s->c_slopes.ensure(std::max<size_t>(1, na) * kmax * sizeof(double))) return e;
   if (int e = s->c_fallback.ensure(std::max<size_t>(1, nt) * sizeof(int))) return e;
-  if (int e = s->c_count.ensure(sizeof(int))) return e;
+  if (int e = s->c_count.ensure(2 * sizeof(unsigned long long))) return e;  // fallback count, evaluations
   MD_HIP(hipMemcpyAsync(s->c_anchors.p, anchors.data(), na * sizeof(int), hipMemcpyHostToDevice, st));
   if (nt) MD_HIP(hipMemcpyAsync(s->c_targets.p, targets.data(), nt * sizeof(mdr::TargetPlan), hipMemcpyHostToDevice, st));
-  const int zero = 0;
-  MD_HIP(hipMemcpyAsync(s->c_count.p, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+  const unsigned long long zero[2] = {0ull, 0ull};
+  MD_HIP(hipMemcpyAsync(s->c_count.p, zero, sizeof(zero), hipMemcpyHostToDevice, st));
   MD_HIP(mdr::launch_gather(s->traces.as<mdr::TraceInfo>(), s->c_anchors.as<int>(), na,
                             s->c_list.as<mdr::TraceInfo>(), st));
   Work wa;
@@ -448,13 +453,17 @@ int run_continuation(md_rocm_solver *s, int kmax, int max_rows, int wave, double
   c.prediction = prediction;
   c.fallback = s->c_fallback.as<int>();
   c.fallback_count = s->c_count.as<int>();
+  c.evaluations = s->c_count.as<unsigned long long>() + 1;
   const int pb = s->cus * (s->cfg.consumer_blocks > 0 ? s->cfg.consumer_blocks : 8);
   MD_HIP(mdr::launch_continuation(c, pb, st));
-  int nfb = 0;
-  MD_HIP(hipMemcpyAsync(&nfb, s->c_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  unsigned long long tally[2] = {0ull, 0ull};
+  MD_HIP(hipMemcpyAsync(tally, s->c_count.p, sizeof(tally), hipMemcpyDeviceToHost, st));
   MD_HIP(hipStreamSynchronize(st));
+  const int nfb = static_cast<int>(tally[0] & 0xffffffffull);
+  s->cont_targets = nt;
   s->cont_accepted = nt - nfb;
   s->cont_fallbacks = nfb;
+  s->cont_evaluations = static_cast<long long>(tally[1]);
   if (nfb > 0) {
     MD_HIP(mdr::launch_gather(s->traces.as<mdr::TraceInfo>(), s->c_fallback.as<int>(), nfb,
                               s->c_list.as<mdr::TraceInfo>(), st));
@@ -680,17 +689,27 @@ int md_rocm_last_stats(md_rocm_solver *s, uint64_t *stats, int n) {
     fail(2, "null solver or stats");
     return -1;
   }
-  const int k = std::min(n, kStatCount);
-  if (!s->stats.p) {
-    for (int i = 0; i < k; ++i) stats[i] = 0;
-    return k;
-  }
-  unsigned long long st[kStatCount];
-  if (hipMemcpyAsync(st, s->stats.p, sizeof(st), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
-      hipStreamSynchronize(s->stream) != hipSuccess) {
+  const int k = std::min(n, kStatCount + 4);
+  unsigned long long st[kStatCount + 4] = {};
+  int invalid = 0;
+  if (s->stats.p && hipMemcpyAsync(st, s->stats.p, kStatCount * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   s->stream) != hipSuccess) {
     fail(1, "stats copy failed");
     return -1;
   }
+  if (s->invalid_live && hipMemcpyAsync(&invalid, s->invalid.p, sizeof(int), hipMemcpyDeviceToHost, s->stream) !=
+                             hipSuccess) {
+    fail(1, "stats copy failed");
+    return -1;
+  }
+  if (hipStreamSynchronize(s->stream) != hipSuccess) {
+    fail(1, "stats copy failed");
+    return -1;
+  }
+  st[kStatCount] = static_cast<unsigned long long>(invalid);
+  st[kStatCount + 1] = static_cast<unsigned long long>(s->cont_targets);
+  st[kStatCount + 2] = static_cast<unsigned long long>(s->cont_accepted);
+  st[kStatCount + 3] = static_cast<unsigned long long>(s->cont_evaluations);
   for (int i = 0; i < k; ++i) stats[i] = st[i];
   return k;
 }
@@ -731,6 +750,7 @@ int md_rocm_adjoint(md_rocm_solver *s, const double *thickness, const double *vp
   return 0;
 }
 
+extern "C++" {
 namespace {
 // Prep + pipeline of a device batch on s->stream (eager, or being captured).
 int device_sequence(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
@@ -750,7 +770,94 @@ int device_sequence(md_rocm_solver *s, const double *thickness, const double *vp
   return run_pipeline(s, w, ntraces, s->nmodes + 2, s->L, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
                       s->cfg.group_velocity ? group : nullptr, group_flags);
 }
+
+// A device-resident launch sequence, replayed from the solver's hipGraph when
+// `key` equals the arguments it was captured for. Otherwise the sequence runs
+// eagerly and, with cfg.graph, is then recorded once more (every buffer is
+// allocated by the eager run) on the solver's own stream for later calls.
+template <class Sequence>
+int graph_run(md_rocm_solver *s, const std::vector<uintptr_t> &key, Sequence &&sequence) {
+  s->timed = true;
+  if (s->cfg.graph && s->graph_exec && key == s->graph_key) {
+    MD_HIP(hipEventRecord(s->ev_begin, s->stream));
+    MD_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+    MD_HIP(hipEventRecord(s->ev_end, s->stream));
+    s->marked_chunks = 0;
+    return 0;
+  }
+  MD_HIP(hipEventRecord(s->ev_begin, s->stream));
+  if (int e = sequence()) return e;
+  MD_HIP(hipEventRecord(s->ev_end, s->stream));
+  if (!s->cfg.graph) return 0;
+  if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+  if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+  s->graph_key.clear();
+  hipStream_t issue = s->stream;
+  s->stream = s->own_stream;
+  s->capturing = true;
+  const hipError_t err = hipStreamBeginCapture(s->own_stream, hipStreamCaptureModeRelaxed);
+  const int e = err == hipSuccess ? sequence() : 1;
+  hipGraph_t g = nullptr;
+  const hipError_t end = err == hipSuccess ? hipStreamEndCapture(s->own_stream, &g) : err;
+  s->capturing = false;
+  s->stream = issue;
+  if (e) {
+    if (g) (void)hipGraphDestroy(g);
+    return e;
+  }
+  if (end != hipSuccess) return fail(1, std::string("graph capture: ") + hipGetErrorString(end));
+  s->graph = g;
+  MD_HIP(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+  s->graph_key = key;
+  return 0;
+}
+
+uintptr_t double_bits(double x) {
+  uintptr_t b;
+  std::memcpy(&b, &x, sizeof(b));
+  return b;
+}
+
+uintptr_t addr(const void *p) { return reinterpret_cast<uintptr_t>(p); }
+
+// Times the launch-shape candidates (traces per scan block x consumer blocks
+// per CU) with `run` (an eager device run) and keeps the fastest.
+template <class Run>
+int autotune_tiles(md_rocm_solver *s, Run &&run, int *scan_waves, int *consumer_blocks) {
+  const md_rocm_config saved = s->cfg;
+  s->cfg.graph = 0;
+  const int waves[] = {1, 2, 4, 8};
+  const int blocks[] = {4, 8, 16};
+  double best = 1e300;
+  int bw = 4, bb = 8;
+  for (int w : waves) {
+    if (w == 8 && !s->cfg.single_precision) continue;
+    for (int b : blocks) {
+      s->cfg.scan_waves = w;
+      s->cfg.consumer_blocks = b;
+      double ms = 0.;
+      for (int rep = 0; rep < 2; ++rep) {  // the first pass warms up, the second is timed
+        if (int e = run()) {
+          s->cfg = saved;
+          return e;
+        }
+        MD_HIP(hipStreamSynchronize(s->stream));
+        float t = 0.f;
+        MD_HIP(hipEventElapsedTime(&t, s->ev_begin, s->ev_end));
+        ms = t;
+      }
+      if (ms < best) { best = ms; bw = w; bb = b; }
+    }
+  }
+  s->cfg = saved;
+  s->cfg.scan_waves = bw;
+  s->cfg.consumer_blocks = bb;
+  if (scan_waves) *scan_waves = bw;
+  if (consumer_blocks) *consumer_blocks = bb;
+  return 0;
+}
 }  // namespace
+}  // extern "C++"
 
 int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
                        const double *rho, int B, const double *periods, int row_periods, const double *bounds,
@@ -768,55 +875,15 @@ int md_rocm_run_device(md_rocm_solver *s, const double *thickness, const double 
   if (int e = s->rows.ensure(static_cast<size_t>(B) * L * sizeof(mdr::PrepRow))) return e;
   if (int e = s->models.ensure(static_cast<size_t>(B) * sizeof(mdr::ModelInfo))) return e;
   if (int e = s->traces.ensure(static_cast<size_t>(ntraces) * sizeof(mdr::TraceInfo))) return e;
-  (void)ntraces;
-  double tolbits = c_tol_rel;
-  uintptr_t tb;
-  std::memcpy(&tb, &tolbits, sizeof(tb));
   const std::vector<uintptr_t> key = {
-      reinterpret_cast<uintptr_t>(thickness), reinterpret_cast<uintptr_t>(vp), reinterpret_cast<uintptr_t>(vs),
-      reinterpret_cast<uintptr_t>(rho), static_cast<uintptr_t>(B), reinterpret_cast<uintptr_t>(periods),
-      static_cast<uintptr_t>(row_periods), reinterpret_cast<uintptr_t>(bounds), tb, static_cast<uintptr_t>(wave),
-      reinterpret_cast<uintptr_t>(velocity), reinterpret_cast<uintptr_t>(flags), reinterpret_cast<uintptr_t>(levels),
-      reinterpret_cast<uintptr_t>(counts), reinterpret_cast<uintptr_t>(group), reinterpret_cast<uintptr_t>(group_flags),
+      0, addr(thickness), addr(vp), addr(vs), addr(rho), static_cast<uintptr_t>(B), addr(periods),
+      static_cast<uintptr_t>(row_periods), addr(bounds), double_bits(c_tol_rel), static_cast<uintptr_t>(wave),
+      addr(velocity), addr(flags), addr(levels), addr(counts), addr(group), addr(group_flags),
       static_cast<uintptr_t>(s->cfg.scan_waves), static_cast<uintptr_t>(s->cfg.consumer_blocks), accept_bits(s)};
-  if (s->cfg.graph && s->graph_exec && key == s->graph_key) {
-    s->timed = true;
-    MD_HIP(hipEventRecord(s->ev_begin, s->stream));
-    MD_HIP(hipGraphLaunch(s->graph_exec, s->stream));
-    MD_HIP(hipEventRecord(s->ev_end, s->stream));
-    s->marked_chunks = 0;
-    return 0;
-  }
-  s->timed = true;
-  MD_HIP(hipEventRecord(s->ev_begin, s->stream));
-  if (int e = device_sequence(s, thickness, vp, vs, rho, B, periods, row_periods, bounds, c_tol_rel, wave, velocity,
-                              flags, levels, counts, group, group_flags))
-    return e;
-  MD_HIP(hipEventRecord(s->ev_end, s->stream));
-  if (s->cfg.graph) {
-    // Record the same sequence (every buffer is now allocated) on the solver's
-    // own stream; later calls with these arguments replay it.
-    if (s->graph_exec) { (void)hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
-    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
-    s->graph_key.clear();
-    hipStream_t issue = s->stream;
-    s->stream = s->own_stream;
-    s->capturing = true;
-    hipError_t err = hipStreamBeginCapture(s->own_stream, hipStreamCaptureModeRelaxed);
-    int e = err == hipSuccess ? device_sequence(s, thickness, vp, vs, rho, B, periods, row_periods, bounds, c_tol_rel,
-                                                wave, velocity, flags, levels, counts, group, group_flags)
-                              : 1;
-    hipGraph_t g = nullptr;
-    const hipError_t end = err == hipSuccess ? hipStreamEndCapture(s->own_stream, &g) : err;
-    s->capturing = false;
-    s->stream = issue;
-    if (e) return e;
-    if (end != hipSuccess) return fail(1, std::string("graph capture: ") + hipGetErrorString(end));
-    s->graph = g;
-    MD_HIP(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
-    s->graph_key = key;
-  }
-  return 0;
+  return graph_run(s, key, [&] {
+    return device_sequence(s, thickness, vp, vs, rho, B, periods, row_periods, bounds, c_tol_rel, wave, velocity,
+                           flags, levels, counts, group, group_flags);
+  });
 }
 
 int md_rocm_run_device_static(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
@@ -853,38 +920,10 @@ int md_rocm_autotune(md_rocm_solver *s, const double *thickness, const double *v
                      int32_t *counts, double *group, uint32_t *group_flags, int *scan_waves,
                      int *consumer_blocks) {
   if (!s) return fail(2, "null solver");
-  const md_rocm_config saved = s->cfg;
-  s->cfg.graph = 0;
-  const int waves[] = {1, 2, 4, 8};
-  const int blocks[] = {4, 8, 16};
-  double best = 1e300;
-  int bw = 4, bb = 8;
-  for (int w : waves) {
-    if (w == 8 && !s->cfg.single_precision) continue;
-    for (int b : blocks) {
-      s->cfg.scan_waves = w;
-      s->cfg.consumer_blocks = b;
-      double ms = 0.;
-      for (int rep = 0; rep < 2; ++rep) {  // the first pass warms up, the second is timed
-        if (int e = md_rocm_run_device(s, thickness, vp, vs, rho, B, periods, row_periods, bounds, c_tol_rel, wave,
-                                       velocity, flags, levels, counts, group, group_flags)) {
-          s->cfg = saved;
-          return e;
-        }
-        MD_HIP(hipStreamSynchronize(s->stream));
-        float t = 0.f;
-        MD_HIP(hipEventElapsedTime(&t, s->ev_begin, s->ev_end));
-        ms = t;
-      }
-      if (ms < best) { best = ms; bw = w; bb = b; }
-    }
-  }
-  s->cfg = saved;
-  s->cfg.scan_waves = bw;
-  s->cfg.consumer_blocks = bb;
-  if (scan_waves) *scan_waves = bw;
-  if (consumer_blocks) *consumer_blocks = bb;
-  return 0;
+  return autotune_tiles(s, [&] {
+    return md_rocm_run_device(s, thickness, vp, vs, rho, B, periods, row_periods, bounds, c_tol_rel, wave, velocity,
+                              flags, levels, counts, group, group_flags);
+  }, scan_waves, consumer_blocks);
 }
 
 int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs, const double *rho,
@@ -1010,32 +1049,33 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
     return fail(2, "null input or output pointer");
   const bool want_group = s->cfg.group_velocity != 0;
   if (want_group && (!group || !group_flags)) return fail(2, "group outputs required");
-  std::vector<mdr::PrepRow> rows(static_cast<size_t>(row_offsets[B]));
-  std::vector<mdr::ModelInfo> models(B);
-  std::vector<mdr::TraceInfo> traces(static_cast<size_t>(period_offsets[B]));
-  const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
-  long long out0 = 0;
-  int kmax = 0, max_rows = 1;
+  if (row_offsets[0] != 0 || period_offsets[0] != 0) return fail(2, "offsets must start at 0");
+  std::vector<int64_t> cells(static_cast<size_t>(B) + 1, 0);
+  int kmax = 0, max_rows = 1, max_modes = 1;
   for (int m = 0; m < B; ++m) {
-    const long long r0 = row_offsets[m], r1 = row_offsets[m + 1];
-    const long long p0 = period_offsets[m], p1 = period_offsets[m + 1];
-    const int L = static_cast<int>(r1 - r0), T = static_cast<int>(p1 - p0);
+    const long long L = row_offsets[m + 1] - row_offsets[m], T = period_offsets[m + 1] - period_offsets[m];
     if (L < 1 || L > mdr::kMaxRows || T < 0 || modes[m] < 1 || modes[m] > 256)
       return fail(2, "invalid ragged geometry at model " + std::to_string(m));
-    for (long long j = r0; j < r1; ++j) rows[j] = mdr::make_prep_row(thickness[j], vp[j], vs[j], rho[j]);
-    double slow;
-    models[m] = mdr::make_model_info(thickness + r0, vp + r0, vs + r0, L, bounds[2 * m], bounds[2 * m + 1], wave,
-                                     modes[m], T, r0, out0, p0, &slow);
-    for (int f = 0; f < T; ++f) traces[p0 + f] = mdr::make_trace(m, f, periods[p0 + f], slow, pp);
-    out0 += static_cast<long long>(modes[m]) * T;
+    cells[m + 1] = cells[m] + static_cast<long long>(modes[m]) * T;
     kmax = std::max(kmax, modes[m] + 2);
-    max_rows = std::max(max_rows, L);
+    max_modes = std::max(max_modes, static_cast<int>(modes[m]));
+    max_rows = std::max(max_rows, static_cast<int>(L));
   }
-  const long long ntr = period_offsets[B];
-  const size_t ncell = static_cast<size_t>(out0);
-  if (int e = s->rows.ensure(rows.size() * sizeof(mdr::PrepRow))) return e;
-  if (int e = s->models.ensure(models.size() * sizeof(mdr::ModelInfo))) return e;
-  if (int e = s->traces.ensure(traces.size() * sizeof(mdr::TraceInfo))) return e;
+  const long long nrows = row_offsets[B], ntr = period_offsets[B];
+  if (nrows > INT32_MAX || ntr > INT32_MAX) return fail(2, "rows and traces must each be below 2^31");
+  const size_t ncell = static_cast<size_t>(cells[B]);
+  // The packed arrays go to the device as they are; the models and traces are
+  // prepared there by the kernels md_rocm_run_ragged_device uses.
+  const size_t nb = static_cast<size_t>(B), r8 = static_cast<size_t>(nrows) * sizeof(double);
+  for (DevBuf *d : {&s->in_h, &s->in_vp, &s->in_vs, &s->in_rho})
+    if (int e = d->ensure(r8)) return e;
+  if (int e = s->in_per.ensure(std::max<size_t>(1, static_cast<size_t>(ntr)) * sizeof(double))) return e;
+  if (int e = s->in_bounds.ensure(2 * nb * sizeof(double))) return e;
+  if (int e = s->in_offsets.ensure(3 * (nb + 1) * sizeof(int64_t) + nb * sizeof(int32_t))) return e;
+  if (int e = s->rows.ensure(static_cast<size_t>(nrows) * sizeof(mdr::PrepRow))) return e;
+  if (int e = s->models.ensure((nb + 1) * sizeof(mdr::ModelInfo))) return e;
+  if (int e = s->traces.ensure(std::max<size_t>(1, static_cast<size_t>(ntr)) * sizeof(mdr::TraceInfo))) return e;
+  if (int e = s->invalid.ensure(sizeof(int))) return e;
   if (int e = s->out_v.ensure(ncell * sizeof(double))) return e;
   if (int e = s->out_flags.ensure(static_cast<size_t>(ntr) * sizeof(uint32_t))) return e;
   if (int e = s->out_levels.ensure(static_cast<size_t>(ntr) * sizeof(int32_t))) return e;
@@ -1047,13 +1087,28 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
   hipStream_t st = s->stream;
   s->timed = true;
   MD_HIP(hipEventRecord(s->ev_begin, st));
-  MD_HIP(hipMemcpyAsync(s->rows.p, rows.data(), rows.size() * sizeof(mdr::PrepRow), hipMemcpyHostToDevice, st));
-  MD_HIP(hipMemcpyAsync(s->models.p, models.data(), models.size() * sizeof(mdr::ModelInfo), hipMemcpyHostToDevice,
-                        st));
-  s->prep_models = B;
+  const std::pair<DevBuf *, const double *> cols[] = {
+      {&s->in_h, thickness}, {&s->in_vp, vp}, {&s->in_vs, vs}, {&s->in_rho, rho}};
+  for (const auto &c : cols) MD_HIP(hipMemcpyAsync(c.first->p, c.second, r8, hipMemcpyHostToDevice, st));
+  if (ntr > 0)
+    MD_HIP(hipMemcpyAsync(s->in_per.p, periods, static_cast<size_t>(ntr) * sizeof(double), hipMemcpyHostToDevice,
+                          st));
+  MD_HIP(hipMemcpyAsync(s->in_bounds.p, bounds, 2 * nb * sizeof(double), hipMemcpyHostToDevice, st));
+  auto *offs = s->in_offsets.as<int64_t>();
+  MD_HIP(hipMemcpyAsync(offs, row_offsets, (nb + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  MD_HIP(hipMemcpyAsync(offs + nb + 1, period_offsets, (nb + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  MD_HIP(hipMemcpyAsync(offs + 2 * (nb + 1), cells.data(), (nb + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  auto *dmodes = reinterpret_cast<int32_t *>(offs + 3 * (nb + 1));
+  MD_HIP(hipMemcpyAsync(dmodes, modes, nb * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  const mdr::RaggedGeometry g{offs, offs + nb + 1, offs + 2 * (nb + 1), dmodes, B, max_rows, max_modes,
+                              nrows, ntr, static_cast<long long>(ncell)};
+  const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
+  MD_HIP(mdr::launch_prep_ragged(s->in_h.as<double>(), s->in_vp.as<double>(), s->in_vs.as<double>(),
+                                 s->in_rho.as<double>(), s->in_bounds.as<double>(), s->in_per.as<double>(), g, pp,
+                                 s->rows.as<mdr::PrepRow>(), s->models.as<mdr::ModelInfo>(),
+                                 s->traces.as<mdr::TraceInfo>(), s->invalid.as<int>(), st));
+  s->prep_models = B + 1;  // the sentinel model B (no valid model leaves traces to it here)
   s->prep_rows = max_rows;
-  MD_HIP(hipMemcpyAsync(s->traces.p, traces.data(), traces.size() * sizeof(mdr::TraceInfo), hipMemcpyHostToDevice,
-                        st));
   std::vector<char> bypassed;
   if (s->cfg.continuation) {
     if (int e = s->c_pred.ensure(static_cast<size_t>(ntr) * sizeof(uint32_t))) return e;
@@ -1106,6 +1161,75 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
 }
 
 }  // extern "C"
+
+int md_rocm_run_ragged_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                              const double *rho, const int64_t *row_offsets, long long nrows, int B,
+                              const double *periods, const int64_t *period_offsets, long long ntraces,
+                              const int32_t *modes, const int64_t *cell_offsets, long long ncells,
+                              const double *bounds, int max_rows, int max_modes, double c_tol_rel, int wave,
+                              double *velocity, uint32_t *flags, int32_t *levels, int32_t *counts, double *group,
+                              uint32_t *group_flags) {
+  if (!s) return fail(2, "null solver");
+  if (B < 1) return fail(2, "B must be positive");
+  if (nrows < 1 || nrows > INT32_MAX) return fail(2, "nrows must be 1..2^31-1");
+  if (ntraces < 0 || ntraces > INT32_MAX) return fail(2, "ntraces must be 0..2^31-1");
+  if (ncells < 0) return fail(2, "ncells must be non-negative");
+  if (max_rows < 1 || max_rows > mdr::kMaxRows)
+    return fail(2, "max_rows must be 1.." + std::to_string(mdr::kMaxRows));
+  if (max_modes < 1 || max_modes > 256) return fail(2, "max_modes must be 1..256");
+  if (wave != 0 && wave != 1) return fail(2, "wave must be 0 (Rayleigh) or 1 (Love)");
+  if (!(c_tol_rel > 0.) || !std::isfinite(c_tol_rel)) return fail(2, "c_tol_rel must be positive");
+  if (!thickness || !vp || !vs || !rho || !row_offsets || !periods || !period_offsets || !modes || !cell_offsets ||
+      !bounds || !velocity || !flags || !levels || !counts)
+    return fail(2, "null input or output pointer");
+  const bool want_group = s->cfg.group_velocity != 0;
+  if (want_group && (!group || !group_flags)) return fail(2, "group outputs required");
+  if (s->cfg.continuation)
+    return fail(2, "anchor continuation plans its anchors from host periods: use md_rocm_run_ragged");
+  if (int e = s->rows.ensure(static_cast<size_t>(nrows) * sizeof(mdr::PrepRow))) return e;
+  if (int e = s->models.ensure(static_cast<size_t>(B + 1) * sizeof(mdr::ModelInfo))) return e;
+  if (int e = s->traces.ensure(static_cast<size_t>(std::max(ntraces, 1LL)) * sizeof(mdr::TraceInfo))) return e;
+  if (int e = s->invalid.ensure(sizeof(int))) return e;
+  const mdr::RaggedGeometry g{row_offsets, period_offsets, cell_offsets, modes, B, max_rows, max_modes,
+                              nrows, ntraces, ncells};
+  const int kmax = max_modes + 2;
+  const std::vector<uintptr_t> key = {
+      1, addr(thickness), addr(vp), addr(vs), addr(rho), addr(row_offsets), static_cast<uintptr_t>(nrows),
+      static_cast<uintptr_t>(B), addr(periods), addr(period_offsets), static_cast<uintptr_t>(ntraces), addr(modes),
+      addr(cell_offsets), static_cast<uintptr_t>(ncells), addr(bounds), static_cast<uintptr_t>(max_rows),
+      static_cast<uintptr_t>(max_modes), double_bits(c_tol_rel), static_cast<uintptr_t>(wave), addr(velocity),
+      addr(flags), addr(levels), addr(counts), addr(group), addr(group_flags),
+      static_cast<uintptr_t>(s->cfg.scan_waves), static_cast<uintptr_t>(s->cfg.consumer_blocks), accept_bits(s)};
+  const int e = graph_run(s, key, [&]() -> int {
+    const mdr::PrepParams pp{wave, s->cfg.tau_floor, s->cfg.tau_phase};
+    MD_HIP(mdr::launch_prep_ragged(thickness, vp, vs, rho, bounds, periods, g, pp, s->rows.as<mdr::PrepRow>(),
+                                   s->models.as<mdr::ModelInfo>(), s->traces.as<mdr::TraceInfo>(),
+                                   s->invalid.as<int>(), s->stream));
+    s->prep_models = B + 1;  // the sentinel model B holds traces no valid model claims
+    s->prep_rows = max_rows;
+    Work w;
+    if (int e = solver_work(s, ntraces, kmax, w)) return e;
+    return run_pipeline(s, w, ntraces, kmax, max_rows, wave, c_tol_rel, velocity, nullptr, flags, levels, counts,
+                        want_group ? group : nullptr, group_flags);
+  });
+  s->invalid_live = e == 0;
+  return e;
+}
+
+int md_rocm_autotune_ragged_device(md_rocm_solver *s, const double *thickness, const double *vp, const double *vs,
+                                   const double *rho, const int64_t *row_offsets, long long nrows, int B,
+                                   const double *periods, const int64_t *period_offsets, long long ntraces,
+                                   const int32_t *modes, const int64_t *cell_offsets, long long ncells,
+                                   const double *bounds, int max_rows, int max_modes, double c_tol_rel, int wave,
+                                   double *velocity, uint32_t *flags, int32_t *levels, int32_t *counts,
+                                   double *group, uint32_t *group_flags, int *scan_waves, int *consumer_blocks) {
+  if (!s) return fail(2, "null solver");
+  return autotune_tiles(s, [&] {
+    return md_rocm_run_ragged_device(s, thickness, vp, vs, rho, row_offsets, nrows, B, periods, period_offsets,
+                                     ntraces, modes, cell_offsets, ncells, bounds, max_rows, max_modes, c_tol_rel,
+                                     wave, velocity, flags, levels, counts, group, group_flags);
+  }, scan_waves, consumer_blocks);
+}
 
 extern "C" int md_rocm_probe_grid(const double *thickness, const double *vp, const double *vs, const double *rho,
                                   int B, int L, const double *periods, int nf, const double *bounds, int nc, int wave,

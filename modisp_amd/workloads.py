@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-__all__ = ["Workload", "precision_batch", "plain_batch", "uneven_batch", "gradient_batch", "sweep_batch",
+__all__ = ["Workload", "precision_batch", "plain_batch", "uneven_batch", "pack_ragged", "gradient_batch", "sweep_batch",
            "monotone_models", "WORKLOADS"]
 
 
@@ -116,6 +116,27 @@ def uneven_batch(B=65536, seed=2):
     modes = rng.integers(1, 11, B)
     return Workload("uneven_batch", h, vp, vs, _density(vp), periods, ("rayleigh",), modes, nfinite + 1,
                     extra=dict(lvz=lvz))
+
+
+def pack_ragged(wl, period_step=1):
+    """Packs a ragged workload (per-model `layers`, period lists and mode
+    counts) into the arrays of the ragged entry points: thickness, vp, vs,
+    rho [rows], row_offsets and period_offsets [B+1], periods, modes [B] and
+    the window bounds [B, 2] = [0.8 min Vs, nextafter(Vs_halfspace, 0)] of
+    each model's own rows. Every `period_step`-th period is kept."""
+    B = wl.B
+    lay = np.asarray(wl.layers if wl.layers is not None else np.full(B, wl.thickness.shape[1]), dtype=np.int64)
+    keep = np.arange(wl.thickness.shape[1])[None, :] < lay[:, None]
+    out = {k: getattr(wl, k)[keep] for k in ("thickness", "vp", "vs", "rho")}
+    per = [np.asarray(p)[::period_step] for p in (wl.periods if isinstance(wl.periods, list) else [wl.periods] * B)]
+    out["row_offsets"] = np.concatenate([[0], np.cumsum(lay)]).astype(np.int64)
+    out["period_offsets"] = np.concatenate([[0], np.cumsum([p.size for p in per])]).astype(np.int64)
+    out["periods"] = np.concatenate(per)
+    out["modes"] = (np.full(B, wl.nmodes) if np.isscalar(wl.nmodes) else np.asarray(wl.nmodes)).astype(np.int32)
+    vs_min = np.where(keep, wl.vs, np.inf).min(axis=1)
+    half = wl.vs[np.arange(B), lay - 1]
+    out["bounds"] = np.stack([0.8 * vs_min, np.nextafter(half, 0.)], axis=1)
+    return out
 
 
 def gradient_batch(B=4096, seed=3):
