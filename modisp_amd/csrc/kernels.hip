@@ -231,7 +231,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
   TraceScan *ts = a.scan + tl;
   constexpr bool fp32 = sizeof(T) == 4;
   if (mi.empty) {
-    if (lane == 0) *ts = TraceScan{0, 0u, mi.empty, 0, 0, 0, 0u, 0u};
+    if (lane == 0) *ts = TraceScan{0, 0u, mi.empty, 0, 0, 0, 0ull, 0ull, 0u};
     return;
   }
   const double lo = mi.lo, hi = mi.hi;
@@ -241,7 +241,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
   // otherwise the window goes straight to the FP64 grid.
   if (fp32 && lo < mi.stable_low && !mi.low_free) {
     if (lane == 0) {
-      *ts = TraceScan{0, kTraceOverflow, 0, 0, 0, 0, 0u, 0u};
+      *ts = TraceScan{0, kTraceOverflow, 0, 0, 0, 0, 0ull, 0ull, 0u};
       a.overflow_list[atomicAdd(a.overflow_count, 1)] = tl;
     }
     return;
@@ -301,7 +301,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
   const double hi_guard = mi.cutoff_guard;
   const float theta = kTroughMax;
   int nb = 0, ne = 0;
-  unsigned umask = 0u, pmask = 0u;
+  uint64_t umask = 0ull, pmask = 0ull;
   bool overflow = false;
   int evaluated = 0;
   for (int base = 0;; base += 62) {
@@ -331,10 +331,45 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
     const bool has_left = j >= 1, has_right = j + 1 < total;
     // Lanes 1..62 see both neighbours; lane 0 only at the first node.
     const bool classify = active && (lane >= 1 || base == 0) && lane <= 62;
+    // A sign change between two nodes outside the FP32 band. FP32 rounding of
+    // the layer constants moves a root by up to ~1e-7 relative, so a node that
+    // close to a root can show the wrong sign. Where the FP32 values cannot
+    // place the root away from both ends, both ends are rechecked in FP64
+    // (uncertain) and the assembly takes the root from whichever cell the
+    // FP64 signs put it in (the change itself, or the cell left or right of
+    // it). That is: a root close to one end (the smaller |F| below 1/50 of
+    // the larger); a step (deep evanescent stacks: F jumps between +-A within
+    // far less than a node spacing, so both ends show nearly the full |F| of
+    // the nodes beyond them, and the step's position is unknown); a cell
+    // narrower than 1e-4 relative (dense nodes next to a layer speed); and a
+    // change at the batch edge, whose far neighbour is not at hand.
+    const bool change = active && has_right && !u && !ur && lane <= 62 && signbit(f) != signbit(fr);
+    const uint64_t cm = __ballot(change);
+    const bool change_l = lane >= 1 && ((cm >> (lane - 1)) & 1);
+    const float f2 = __shfl(fr, (lane + 1) & 63, 64);  // value two nodes to the right
+    bool verify = false;
+    if (fp32 && change) {
+      const float a1 = fabsf(f), a2 = fabsf(fr), amin = fminf(a1, a2), amax = fmaxf(a1, a2);
+      const float outer = fmaxf(has_left ? fabsf(fl) : 0.f, j + 2 < total ? fabsf(f2) : 0.f);
+      verify = lane >= 62 || amin < 0.02f * amax || amin >= 0.5f * outer || cr - c < 1e-4 * c;
+    }
+    const uint64_t vm = __ballot(verify);
+    const bool verify_l = lane >= 1 && ((vm >> (lane - 1)) & 1);
+    // A same-sign trough needs its neighbours' signs: next to an FP32-uncertain
+    // node a local minimum of |F| below the probe threshold is rechecked too,
+    // and the trough test is repeated on the FP64 values (unc_probe_kernel).
+    // Otherwise a hidden pair beside the cutoff guard or the band would be
+    // probed only by the FP64 path.
+    const bool trough_pending = fp32 && !u && (ul || ur) && has_left && has_right && !(cl < mi.stable_low) &&
+                                fabsf(f) < theta && fabsf(f) < fabsf(fl) && fabsf(f) < fabsf(fr);
+    // A change whose right node is rechecked belongs to that node's event
+    // (its left cell), not to a certain bracket here.
+    const bool recheck = u || verify || verify_l || trough_pending;
+    const bool recheck_r = (__ballot(recheck) >> ((lane + 1) & 63)) & 1;
     int kind = kNone, probe = 0;
     if (classify) {
-      if (u) kind = kUncertain;
-      else if (has_right && !ur && signbit(f) != signbit(fr)) kind = kBracket;
+      if (recheck) kind = kUncertain;
+      else if (change && !recheck_r) kind = kBracket;
       // No probe next to FP32 values below the stable limit (meaningless
       // there, and the band holds no root).
       if (has_left && has_right && !(fp32 && cl < mi.stable_low)) {
@@ -348,19 +383,26 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
         const bool same_l = signbit(fl) == signbit(f), same_r = signbit(fr) == signbit(f);
         if (certain3 && same_l && same_r && af < fabsf(fl) && af < fabsf(fr) && af < theta) probe = kTrough;
         else if (sc < sl && sc < sr && 0.5f * (sl + sr) - sc > kDipDepth && !(certain3 && !same_l && !same_r) &&
-                 (a.wave == 0 || (certain3 && same_l && same_r)))
-          probe = kDip;  // Love: two-sided only; its exact phase count catches the rest
+                 (a.wave == 0 || !certain3 || (same_l && same_r)))
+          probe = kDip;  // Love: two-sided where the signs are known; its exact phase count catches the rest
       }
       if (probe && kind == kNone) kind = kProbe;
     }
-    const uint64_t bm = __ballot(kind == kBracket);
+    // Sign changes counted toward the cut at K. FP64: every certain one.
+    // FP32: a change whose neighbouring cells hold none (a lone node of the
+    // wrong sign makes two adjacent changes that its recheck removes; a
+    // change at lane 62 is left to be counted later), so the cut never comes
+    // before K roots the FP64 signs confirm.
+    const bool counted = fp32 ? (change && classify && !change_l && lane <= 61 && !((cm >> (lane + 1)) & 1))
+                              : kind == kBracket;
+    const uint64_t bm = __ballot(counted);
     uint64_t em = __ballot(kind != kNone);
     bool done = total - 1 - base <= 62;
     const int nbb = __popcll(bm);
-    if (nb + nbb >= K) {  // cut at the K-th certain sign change
+    if (nb + nbb >= K) {  // cut at the K-th counted sign change (keeping its right node's event)
       uint64_t m = bm;
       for (int r = nb + 1; r < K; ++r) m &= m - 1;
-      const int cut = __builtin_ctzll(m);
+      const int cut = __builtin_ctzll(m) + (fp32 ? 1 : 0);
       em &= cut == 63 ? ~0ull : ((2ull << cut) - 1ull);
       done = true;
     }
@@ -376,12 +418,12 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
       // order are slots ne, ne + 1, ...; compress the uncertain / probed lane
       // ballots onto those slots (wave-uniform, scalar work).
       const uint64_t ub = __ballot(kind == kUncertain) & em, pb = __ballot(probe != 0) & em;
-      unsigned cu = 0u, cp = 0u;
+      uint64_t cu = 0ull, cp = 0ull;
       int i = 0;
       for (uint64_t m = em; m; m &= m - 1ull, ++i) {
         const uint64_t low = m & (~m + 1ull);
-        if (ub & low) cu |= 1u << i;
-        if (pb & low) cp |= 1u << i;
+        if (ub & low) cu |= 1ull << i;
+        if (pb & low) cp |= 1ull << i;
       }
       umask |= cu << ne;
       pmask |= cp << ne;
@@ -411,7 +453,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
     if (done) break;
   }
   if (lane == 0) {
-    *ts = TraceScan{overflow ? 0 : ne, overflow ? kTraceOverflow : 0u, 0, nb, 0, evaluated, umask, pmask};
+    *ts = TraceScan{overflow ? 0 : ne, overflow ? kTraceOverflow : 0u, 0, nb, 0, evaluated, umask, pmask, 0u};
     // An FP32 trace whose events overflow is rescanned on the FP64 grid,
     // which has no uncertain nodes.
     if (overflow && fp32) a.overflow_list[atomicAdd(a.overflow_count, 1)] = tl;
@@ -565,12 +607,12 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(PipeArgs a) {
   const int tl = blockIdx.x * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool live = tl < a.trace_end - a.trace_begin;
-  unsigned um = 0u, pm = 0u;
+  uint64_t um = 0ull, pm = 0ull;
   if (live) {
     const TraceScan ts = a.scan[tl];
     if (ts.status == 0 && !(ts.flags & kTraceOverflow)) { um = ts.umask; pm = ts.pmask; }
   }
-  const int nu = __popc(um), np = __popc(pm);
+  const int nu = __popcll(um), np = __popcll(pm);
   const int iu = wave_incl_scan(nu, lane), ip = wave_incl_scan(np, lane);
   if (lane == 63) { part[0][w] = iu; part[1][w] = ip; }
   __syncthreads();
@@ -587,8 +629,8 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(PipeArgs a) {
   __syncthreads();
   int qu = base[0] + part[0][w] + iu - nu, qp = base[1] + part[1][w] + ip - np;
   const uint32_t g0 = static_cast<uint32_t>(tl) * kEventCap;
-  for (; um; um &= um - 1u) a.unc_queue[qu++] = g0 + __builtin_ctz(um);
-  for (; pm; pm &= pm - 1u) a.trough_queue[qp++] = g0 + __builtin_ctz(pm);
+  for (; um; um &= um - 1ull) a.unc_queue[qu++] = g0 + __builtin_ctzll(um);
+  for (; pm; pm &= pm - 1ull) a.trough_queue[qp++] = g0 + __builtin_ctzll(pm);
 }
 
 // ---------------------------------------------------------------------------

@@ -18,7 +18,7 @@ namespace mdr {
 
 constexpr int kBlock = 256;       // lanes per scan block (4 wave64)
 constexpr int kMaxRows = 256;     // layer rows per model (reference limit, __init__.py:152)
-constexpr int kEventCap = 32;     // ascending events kept per trace
+constexpr int kEventCap = 64;     // ascending events kept per trace (one bit each in TraceScan's masks)
 constexpr int kStats = 9;         // run statistics (stats_kernel)
 constexpr int kFloor = 64;        // uniform table points of the adaptive grid (one per lane)
 constexpr int kAdaptRows = 64;    // rows of the common scan scratch; deeper models use a kMaxRows scratch
@@ -117,7 +117,7 @@ struct TraceScan {
   int certain;  // certain sign changes seen by the scan
   int count;    // roots found (<= K), -1 unresolved
   int nodes;    // secular evaluations of the trace's scan (lane slots, batch overlaps included)
-  unsigned umask, pmask;  // events (bit e) that are FP32-uncertain / probed for a hidden pair
+  uint64_t umask, pmask;  // events (bit e) that are FP32-uncertain / probed for a hidden pair
   unsigned rejected;      // reported modes (bit m) whose |F| failed the f_accept threshold
 };
 

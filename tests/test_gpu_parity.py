@@ -354,30 +354,6 @@ def test_full_precision_workload_fp32_and_fp64_paths(md, wave):
     assert np.all((v >= lo)[fin]) and np.all((v <= hi)[fin])
 
 
-def test_fp32_grid_signs_outside_the_band(md):
-    """Wherever an FP32 grid value is outside the uncertainty band the pipeline
-    trusts its sign; it must equal the FP64 sign."""
-    from modisp_amd import workloads
-    wl = workloads.precision_batch(B=48, seed=106)
-    win = workloads.scan_window(wl.vs)
-    slow = np.sum(wl.thickness[:, :-1] / wl.vs[:, :-1], axis=1)
-    phi = (2 * np.pi / wl.periods)[None, :] * slow[:, None]
-    tau = np.maximum(1e-4, 5e-5 * (1 + phi))[:, :, None]
-    c = win[:, 0, None] + (win[:, 1] - win[:, 0])[:, None] * np.arange(512)[None, :] / 511.
-    guard = (c >= wl.vs[:, -1, None] * (1 - 1e-3))[:, None, :]
-    for wave in (0, 1):
-        f32 = md.probe_grid(wl.thickness, wl.vp, wl.vs, wl.rho, wl.periods, win, 512, wave, True)
-        f64 = md.probe_grid(wl.thickness, wl.vp, wl.vs, wl.rho, wl.periods, win, 512, wave, False)
-        trusted = (np.abs(f32) >= tau) & ~guard
-        assert np.array_equal(np.signbit(f32)[trusted], np.signbit(f64)[trusted])
-        # FP64 grid values agree in sign with the oracle's released evaluator.
-        for i in range(0, wl.B, 12):
-            for f in range(0, wl.periods.size, 25):
-                ref = oracle.secular(*(getattr(wl, k)[i] for k in MODEL), wl.periods[f], c[i], wave)
-                sure = np.abs(ref) > 1e-12
-                assert np.array_equal(np.signbit(ref)[sure], np.signbit(f64[i, f])[sure])
-
-
 @pytest.mark.parametrize("method", [0, 1, 2], ids=["legacy", "brent", "chandrupatla"])
 def test_refinement_methods_match_the_oracle(md, method):
     """Each refinement the reference offers (QeSolver refinement 0 legacy
