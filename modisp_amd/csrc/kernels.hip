@@ -326,6 +326,13 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
     const float fl = __shfl(f, (lane + 63) & 63, 64), fr = __shfl(f, (lane + 1) & 63, 64);
     const float sl = __shfl(sc, (lane + 63) & 63, 64), sr = __shfl(sc, (lane + 1) & 63, 64);
     const double cl = __shfl(c, (lane + 63) & 63, 64), cr = __shfl(c, (lane + 1) & 63, 64);
+    // Nodes closer than 1e-6 (relative) to a neighbour -- the quadratic
+    // spacing after a layer speed packs them there when many modes crowd
+    // above it -- are below what FP32 resolves (its k steps by 6e-8; snapped
+    // nodes can even coincide, and at the speed itself FP32 takes the r -> 0
+    // limit where FP64 does not): they are rechecked.
+    if (fp32 && active && ((lane >= 1 && j >= 1 && c - cl < 1e-6 * c) || (lane <= 62 && j + 1 < total && cr - c < 1e-6 * c)))
+      u = true;
     const uint64_t um = __ballot(u);
     const bool ul = (um >> ((lane + 63) & 63)) & 1, ur = (um >> ((lane + 1) & 63)) & 1;
     const bool has_left = j >= 1, has_right = j + 1 < total;
@@ -351,8 +358,13 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
     if (fp32 && change) {
       const float a1 = fabsf(f), a2 = fabsf(fr), amin = fminf(a1, a2), amax = fmaxf(a1, a2);
       const float outer = fmaxf(has_left ? fabsf(fl) : 0.f, j + 2 < total ? fabsf(f2) : 0.f);
-      verify = lane >= 62 || amin < 0.02f * amax || amin >= 0.5f * outer || cr - c < 1e-4 * c;
+      verify = lane >= 62 || amin < 0.02f * amax || amin >= 0.25f * outer || cr - c < 1e-4 * c;
     }
+    // A change beside a scale dip (the state collapses there, so F is as
+    // sensitive to rounding as at a step) is rechecked too.
+    const bool dip_here = has_left && has_right && sc < sl && sc < sr && 0.5f * (sl + sr) - sc > kDipDepth;
+    const bool dip_right = (__ballot(dip_here) >> ((lane + 1) & 63)) & 1;
+    verify = verify || (fp32 && change && (dip_here || dip_right));
     const uint64_t vm = __ballot(verify);
     const bool verify_l = lane >= 1 && ((vm >> (lane - 1)) & 1);
     // A same-sign trough needs its neighbours' signs: next to an FP32-uncertain
@@ -361,10 +373,13 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
     // Otherwise a hidden pair beside the cutoff guard or the band would be
     // probed only by the FP64 path.
     const bool trough_pending = fp32 && !u && (ul || ur) && has_left && has_right && !(cl < mi.stable_low) &&
-                                fabsf(f) < theta && fabsf(f) < fabsf(fl) && fabsf(f) < fabsf(fr);
+                                fabsf(f) < fabsf(fl) && fabsf(f) < fabsf(fr) &&
+                                (fabsf(f) < theta || fabsf(f) < kTroughRel * fminf(fabsf(fl), fabsf(fr)));
     // A change whose right node is rechecked belongs to that node's event
     // (its left cell), not to a certain bracket here.
-    const bool recheck = u || verify || verify_l || trough_pending;
+    // A node at a scale dip is rechecked as well: where the state collapses
+    // F steps, so its FP32 sign decides which cells a pair probe searches.
+    const bool recheck = u || verify || verify_l || trough_pending || (fp32 && dip_here);
     const bool recheck_r = (__ballot(recheck) >> ((lane + 1) & 63)) & 1;
     int kind = kNone, probe = 0;
     if (classify) {
@@ -381,10 +396,29 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
         const float af = fabsf(f);
         const bool certain3 = !u && !ul && !ur;
         const bool same_l = signbit(fl) == signbit(f), same_r = signbit(fr) == signbit(f);
-        if (certain3 && same_l && same_r && af < fabsf(fl) && af < fabsf(fr) && af < theta) probe = kTrough;
-        else if (sc < sl && sc < sr && 0.5f * (sl + sr) - sc > kDipDepth && !(certain3 && !same_l && !same_r) &&
-                 (a.wave == 0 || !certain3 || (same_l && same_r)))
-          probe = kDip;  // Love: two-sided where the signs are known; its exact phase count catches the rest
+        // A same-sign local minimum of |F| that is small, or 10% below both
+        // neighbours: F falling toward zero and rising again without a sign
+        // change is the trace of two roots closer than the node spacing, and
+        // a pair in a broad trough can leave no node below theta. (Steps of F
+        // in deep evanescent stacks vary by far less between nodes.)
+        // A scale dip, where there is one, leads the search unless |F| itself
+        // is below theta: the scale follows a pair more surely than a shallow
+        // trough of F does.
+        const bool low3 = certain3 && same_l && same_r && af < fabsf(fl) && af < fabsf(fr);
+        if (low3 && af < theta)
+          probe = kTrough;
+        else if (sc < sl && sc < sr && 0.5f * (sl + sr) - sc > kDipDepth && !(certain3 && !same_l && !same_r))
+          probe = kDip;  // both wave types: a Love trace whose phase count is unavailable relies on it too
+        else if (low3 && af < kTroughRel * fminf(fabsf(fl), fabsf(fr)))
+          probe = kTrough;
+      } else if ((has_left != has_right) && !fp32) {
+        // A window end: a pair in the end cell shows only as |F| falling
+        // toward the end (one side). The FP32 ends are rechecked in FP64 and
+        // take this test on those values (unc_probe_kernel).
+        const float an = has_left ? fabsf(fl) : fabsf(fr);
+        const bool same = has_left ? signbit(fl) == signbit(f) : signbit(fr) == signbit(f);
+        const float af = fabsf(f);
+        if (same && af < an && (af < theta || af < kTroughRel * an)) probe = kTrough;
       }
       if (probe && kind == kNone) kind = kProbe;
     }
@@ -688,10 +722,30 @@ __global__ __launch_bounds__(kBlock) void unc_probe_kernel(PipeArgs a) {
       const Event &x = ev[e];
       // The FP32 probe pass cannot work below the FP32 limit (a root-free band).
       const double low = a.fp32 ? a.models[a.traces[tl + a.trace_begin].model].stable_low : 0.;
-      if (!x.probe && x.c0 == x.c0 && x.c2 == x.c2 && !(x.c0 < low)) {
+      const int kind = x.probe & 0xff;
+      const bool has_l = x.c0 == x.c0, has_r = x.c2 == x.c2;
+      if (kind != kTrough && (has_l || has_r) && !(has_l && x.c0 < low) && !(!has_l && x.c1 < low)) {
         const NodeValues v = node_values(ev, e, a.scan[tl].nev);
-        want = signbit(v.l) == signbit(v.c) && signbit(v.r) == signbit(v.c) && fabs(v.c) < fabs(v.l) &&
-               fabs(v.c) < fabs(v.r) && fabs(v.c) < double(kTroughMax);
+        // Both neighbours, or the one neighbour of a window end.
+        const double al = has_l ? fabs(v.l) : INFINITY, ar = has_r ? fabs(v.r) : INFINITY;
+        const bool falls = (!has_l || signbit(v.l) == signbit(v.c)) && (!has_r || signbit(v.r) == signbit(v.c)) &&
+                           fabs(v.c) < al && fabs(v.c) < ar;
+        const bool trough_abs = falls && fabs(v.c) < double(kTroughMax);
+        // A scale dip the scan flagged at an uncertain node is already queued;
+        // where the FP64 values show |F| below theta there it is searched as a
+        // trough (on sign * F, as the FP64 scan would have flagged it).
+        if (trough_abs && kind == kDip) a.events[g].probe = (x.probe & ~0xff) | kTrough;
+        // A scale dip the scan left unprobed because the FP32 signs changed on
+        // both sides (a lone FP32 sign error there) is probed once the FP64
+        // signs say otherwise, as the FP64 scan would have flagged it; a
+        // relative trough only where there is no dip to follow.
+        const bool same_l = signbit(v.l) == signbit(v.c), same_r = signbit(v.r) == signbit(v.c);
+        const bool dip = kind == 0 && !trough_abs && has_l && has_r && x.sc < x.sl && x.sc < x.sr &&
+                         0.5f * (x.sl + x.sr) - x.sc > kDipDepth && (same_l || same_r);
+        const bool trough_rel = kind == 0 && !trough_abs && !dip && falls &&
+                                fabs(v.c) < double(kTroughRel) * fmin(al, ar);
+        if (dip) a.events[g].probe = (x.probe & ~0xff) | kDip;
+        want = kind == 0 && (trough_abs || dip || trough_rel);
       }
     }
     const uint64_t m = __ballot(want);
@@ -700,7 +754,7 @@ __global__ __launch_bounds__(kBlock) void unc_probe_kernel(PipeArgs a) {
     if (lane == 0) slot = atomicAdd(a.trough_count, __popcll(m));
     slot = __shfl(slot, 0, 64) + __popcll(m & lanes_below(lane));
     if (want) {
-      a.events[g].probe = kTrough;
+      if ((a.events[g].probe & 0xff) == 0) a.events[g].probe = kTrough;
       if (slot < a.queue_cap) a.trough_queue[slot] = g;
     }
   }
@@ -835,6 +889,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
       }
       double split, fsplit = 0.;
       if (!golden_step(s, split, fsplit)) break;  // needs F at s.pending
+      if (fp32 && split == split) {
+        // An FP32 point of opposite sign can sit on the wrong side of a root
+        // that close (FP32 moves a step of F by ~1e-7): the FP64 pass repeats
+        // the search on the current bracket, so the split it keeps is an FP64
+        // sign change. Rare, so one atomic per lane.
+        Event &x = a.events[g];
+        x.x = s.lo;
+        x.vx = s.hi;
+        x.pad = __float_as_int(s.stage == 2 ? s.kmin : INFINITY);
+        x.probe |= kProbeNeeds64 | (s.it << 16);
+        const int slot = atomicAdd(a.handoff_count, 1);
+        if (slot < a.queue_cap) a.handoff_list[slot] = g;
+        active = false;
+        continue;
+      }
       a.events[g].x = split;
       a.events[g].vx = fsplit;
       active = false;

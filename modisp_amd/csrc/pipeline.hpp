@@ -23,7 +23,8 @@ constexpr int kStats = 9;         // run statistics (stats_kernel)
 constexpr int kFloor = 64;        // uniform table points of the adaptive grid (one per lane)
 constexpr int kAdaptRows = 64;    // rows of the common scan scratch; deeper models use a kMaxRows scratch
 constexpr int kCellMax = 1 << 16; // density nodes per table cell (cap)
-constexpr float kTroughMax = 0.05f;  // |F| below which a same-sign trough is probed
+constexpr float kTroughMax = 0.05f;  // |F| below which a same-sign trough is probed ...
+constexpr float kTroughRel = 0.9f;   // ... or below this share of both neighbours' |F| (a trough at any height)
 constexpr int kTroughIters = 28;  // golden-section steps per trough / dip probe
 constexpr float kDipDepth = 0.5f; // log2-scale dip (bits below the neighbours' mean) that is probed
 constexpr double kGuardTol = 1e-7;  // relative tolerance of the guard roots (modes+1, modes+2)
