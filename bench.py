@@ -14,7 +14,10 @@ FP64 path (identical grid and pipeline, every scan evaluation in FP64) and,
 for reference, for the literal 512-sample uniform grid in both precisions.
 
 `graph` is the same device-resident step replayed from one captured hipGraph
-per wave type. `end_to_end` follows the reference's own timing protocol:
+per wave type. HIP runs the replays of both solvers' graphs on one internal
+queue, so the two wave types' replays run one after the other where their eager
+launches overlap on two streams; the leg therefore also reports each wave
+type's replay device time beside its eager pass. `end_to_end` follows the reference's own timing protocol:
 Solver calls on host arrays (copies and host result assembly included), median
 of the steady calls, valid roots = status-0 cells.
 
@@ -641,11 +644,15 @@ def main():
     if not args.no_graph:
         # The same device-resident step replayed from one captured hipGraph per
         # wave type (captured on the first warmup call).
-        cases, wall, _, roots, _ = run_path("float32", "adaptive", graph=True)
+        cases, wall, gst, roots, _ = run_path("float32", "adaptive", graph=True)
         for c in cases:
             c.close()
         result["graph"] = {"value": roots * args.steps / wall, "unit": "roots/s",
-                           "ms_per_step": wall / args.steps * 1e3, "valid_roots": roots}
+                           "ms_per_step": wall / args.steps * 1e3, "valid_roots": roots,
+                           "replay_device_ms": {"rayleigh": float(gst[0][0]), "love": float(gst[1][0])},
+                           "eager_device_ms": {"rayleigh": float(st32[0][0]), "love": float(st32[1][0])},
+                           "note": "both solvers' replays share one HIP-internal queue (no overlap of the wave "
+                                   "types); per wave type, replay device time vs the eager pass"}
     if not args.no_e2e:
         result["end_to_end"] = end_to_end_leg(wl, args.steps)
         log(f"end to end: {result['end_to_end']['value']:.3g} roots/s (Solver calls on host arrays)")

@@ -60,7 +60,7 @@ typedef struct md_rocm_config {
                               with identical arguments */
   int scan_waves;          /* traces (wavefronts) per scan block: 1, 2, 4 or 8 (FP64: <= 4);
                               0 = 4, or what md_rocm_autotune picked */
-  int consumer_blocks;     /* persistent consumer blocks per compute unit, 1..64 (0 = 8) */
+  int consumer_blocks;     /* persistent consumer blocks per compute unit, 1..64 (0 = 16) */
   int refinement;          /* bracket refinement (reference QeSolver refinement, refinement.hpp:11-65):
                               0 legacy regula falsi, 1 Brent-Dekker (default), 2 Chandrupatla */
   int shape;               /* reference `shape` (search_strategy shape_adaptive, the default 1, or

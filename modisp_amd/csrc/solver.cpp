@@ -63,6 +63,17 @@ struct DevBuf {
 
 constexpr int kStatCount = mdr::kStats;  // see md_rocm_last_stats
 
+// Persistent consumer blocks per CU when the configuration leaves it at 0.
+// More blocks than a CU holds resident at once: a block that ends early is
+// replaced by a waiting one, which evens out the uneven per-block work of the
+// lane-persistent kernels (16 measured fastest on the precision workload's
+// recheck, probe and refinement kernels among 4..48).
+constexpr int kDefaultConsumerBlocks = 16;
+
+int consumer_blocks_per_cu(const md_rocm_config &c) {
+  return c.consumer_blocks > 0 ? c.consumer_blocks : kDefaultConsumerBlocks;
+}
+
 }  // namespace
 
 // The fixed device workspace of md_rocm_run_device_static: one array per GPU,
@@ -301,7 +312,7 @@ int run_pipeline(md_rocm_solver *s, const Work &w, long long ntraces, int kmax, 
   a.group = group;
   a.group_flags = gflags;
   a.stats = w.stats;
-  const int persistent = s->cus * (s->cfg.consumer_blocks > 0 ? s->cfg.consumer_blocks : 8);
+  const int persistent = s->cus * consumer_blocks_per_cu(s->cfg);
   const int nchunks = static_cast<int>((ntraces + chunk - 1) / chunk);
   s->marked_chunks = w.marks ? nchunks : 0;
   int c = 0;
@@ -454,7 +465,7 @@ int run_continuation(md_rocm_solver *s, int kmax, int max_rows, int wave, double
   c.fallback = s->c_fallback.as<int>();
   c.fallback_count = s->c_count.as<int>();
   c.evaluations = s->c_count.as<unsigned long long>() + 1;
-  const int pb = s->cus * (s->cfg.consumer_blocks > 0 ? s->cfg.consumer_blocks : 8);
+  const int pb = s->cus * consumer_blocks_per_cu(s->cfg);
   MD_HIP(mdr::launch_continuation(c, pb, st));
   unsigned long long tally[2] = {0ull, 0ull};
   MD_HIP(hipMemcpyAsync(tally, s->c_count.p, sizeof(tally), hipMemcpyDeviceToHost, st));
@@ -608,7 +619,7 @@ static int fill_report(int which, int L, int single_precision, const md_rocm_con
   out->dynamic_shared_bytes = r.dynamic_shared;
   out->search_block_threads = r.block;
   out->max_active_blocks_per_cu = r.active_per_cu;
-  out->configured_blocks_per_cu = cfg && cfg->consumer_blocks > 0 ? cfg->consumer_blocks : 8;
+  out->configured_blocks_per_cu = cfg ? consumer_blocks_per_cu(*cfg) : kDefaultConsumerBlocks;
   out->table_capacity_per_model = r.table_rows;
   out->cache_points_per_trace = r.events_per_trace;
   return 0;
@@ -827,9 +838,9 @@ int autotune_tiles(md_rocm_solver *s, Run &&run, int *scan_waves, int *consumer_
   const md_rocm_config saved = s->cfg;
   s->cfg.graph = 0;
   const int waves[] = {1, 2, 4, 8};
-  const int blocks[] = {4, 8, 16};
+  const int blocks[] = {8, 16, 24};
   double best = 1e300;
-  int bw = 4, bb = 8;
+  int bw = 4, bb = kDefaultConsumerBlocks;
   for (int w : waves) {
     if (w == 8 && !s->cfg.single_precision) continue;
     for (int b : blocks) {

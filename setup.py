@@ -20,5 +20,7 @@ setup(
     package_dir={"modisp_rocm": "modisp_amd"},
     packages=["modisp_rocm"],
     package_data={"modisp_rocm": ["libmodisp_rocm.so", "csrc/*.hip", "csrc/*.hpp", "csrc/*.cpp"]},
+    entry_points={"console_scripts": ["modisp-rocm=modisp_rocm.cli:main",
+                                      "modisp-rocm-benchmark=modisp_rocm.benchmark:main"]},
     zip_safe=False,
 )

@@ -83,7 +83,7 @@ def test_autotuner_picks_a_candidate_and_shapes_agree(env):
         s.synchronize()
         ref = _snapshot(bufs)
         w, b = s.autotune_device_batch(B, ptr, 1e-12, 0)
-        assert w in (1, 2, 4, 8) and b in (4, 8, 16)
+        assert w in (1, 2, 4, 8) and b in (8, 16, 24)
         s.run_device_batch(B, ptr, 1e-12, 0)
         s.synchronize()
         _equal(_snapshot(bufs), ref)
@@ -169,7 +169,7 @@ def test_ragged_device_autotuner_and_inconsistent_offsets(env):
     ref = _host_ragged(md, packed, geom)
     with md.QeSolver(geom["B"], geom["max_rows"], 1, 1, group_velocity=True) as s:
         w, b = s.autotune_ragged_device_batch(geom, ptr, 1e-12, 0)
-        assert w in (1, 2, 4, 8) and b in (4, 8, 16)
+        assert w in (1, 2, 4, 8) and b in (8, 16, 24)
         s.run_ragged_device_batch(geom, ptr, 1e-12, 0)
         s.synchronize()
         _ragged_equal(bufs, ref)

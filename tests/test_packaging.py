@@ -25,6 +25,10 @@ def test_wheel_installs_modisp_rocm(tmp_path):
     subprocess.run([sys.executable, "-m", "pip", "install", "-q", "--no-deps", "--no-index", "--target", str(site),
                     str(wheel[0])], check=True)
     assert (site / "modisp_rocm" / "libmodisp_rocm.so").exists()
+    entry = next(site.glob("modisp_rocm-0.5.0.dist-info")) / "entry_points.txt"
+    scripts = entry.read_text()
+    assert "modisp-rocm = modisp_rocm.cli:main" in scripts
+    assert "modisp-rocm-benchmark = modisp_rocm.benchmark:main" in scripts
     probe = ("import sys; sys.path.insert(0, sys.argv[1]); import modisp_rocm; m = modisp_rocm.load(); "
              "assert m is modisp_rocm and m.VERSION == '0.5.0' and m.ABI_VERSION == 9; "
              "assert m.SUPPORTS_FP32 and m.SUPPORTS_GROUP_VELOCITY and m.SUPPORTS_CONTINUATION; "
