@@ -156,7 +156,7 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 // The phase velocity whose wavenumber omega/c is exactly representable in
 // FP32: evaluating F in FP32 at k = float(omega / c) evaluates it at this c.
 // The result is kept inside [lo, hi] (one FP32 step of k inward).
-__device__ __forceinline__ double snap32(double omega, double c, double lo, double hi) {
+__device__ __forceinline__ double snap32(double omega, double c, double lo, double hi, float *k_out = nullptr) {
   float k = float(omega / c);
   double s = omega / double(k);
   if (s > hi) {
@@ -166,6 +166,9 @@ __device__ __forceinline__ double snap32(double omega, double c, double lo, doub
     k = __int_as_float(__float_as_int(k) - 1);  // next float down
     s = omega / double(k);
   }
+  // float(omega / s) rounds back to k exactly (the double quotient is within
+  // 2^-52 of k, far inside half a float ulp), so callers may take k as is.
+  if (k_out) *k_out = k;
   return s;
 }
 
@@ -317,8 +320,9 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
       // the FP32 k actually evaluated, so a node's sign belongs to its
       // coordinate. The window ends stay exact and are always rechecked in FP64.
       const bool end = j == 0 || j == total - 1;
-      if (fp32 && !end) c = snap32(omega, c, lo, hi);
-      const T k = T(omega / c);
+      float k32 = 0.f;
+      if (fp32 && !end) c = snap32(omega, c, lo, hi, &k32);
+      const T k = fp32 && !end ? T(k32) : T(omega / c);
       const T F = a.wave == 0 ? rayleigh_F_scale<T>(sh.lay, L, top, k, &sc) : love_F_scale<T>(sh.lay, L, top, k, &sc);
       f = float(F);
       u = fp32 && (!(fabsf(f) >= tau) || c >= hi_guard || end || c < mi.stable_low);
@@ -821,14 +825,21 @@ __device__ __forceinline__ void golden_take(GoldenState &s, double x, double f, 
 // best key (float bits), until the FP64 pass writes the split.
 constexpr int kProbeNeeds64 = 0x100;
 
-// Probes of same-sign troughs and scale dips, one lane per probe with
-// lane-persistent scheduling. T = float: the FP32 pass of the FP32 path -- when
-// a point falls inside the FP32 uncertainty band (or the cutoff guard, or the
-// bracket has narrowed to the FP32 spacing of k) the search is handed, with its
-// bracket, iteration count and best key, to the FP64 pass. T = double: every
-// probe of the FP64 path, or the handed-over probes of the FP32 path, in FP64;
-// a resumed search restarts the golden section on the handed-over bracket (one
+// Probes of same-sign troughs and scale dips, one search per lane as a state
+// machine. T = float: the FP32 pass of the FP32 path -- when a point falls
+// inside the FP32 uncertainty band (or the cutoff guard, or the bracket has
+// narrowed to the FP32 spacing of k) the search is handed, with its bracket,
+// iteration count and best key, to the FP64 pass. T = double: every probe of
+// the FP64 path, or the handed-over probes of the FP32 path, in FP64; a
+// resumed search restarts the golden section on the handed-over bracket (one
 // extra evaluation: the retained interior point sits at a golden point of it).
+//
+// Work order: the probe list is cut into runs of 64 consecutive entries (the
+// events of neighbouring traces, in ascending c); a wave takes the runs wave,
+// wave + waves, ... as one stream and hands its entries out in order to
+// whichever lanes are idle (ballot + popcount prefix), so the searches in
+// flight in a wave stay within about one run: similar traces and speeds, so
+// the branches of the wave's evaluations mostly agree.
 template <class T, int Wave>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void trough_kernel(PipeArgs a) {
   constexpr bool fp32 = sizeof(T) == 4;
@@ -836,8 +847,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   const bool from_list = !fp32 && a.fp32;
   const int n = min(from_list ? *a.handoff_count : *a.trough_count, a.queue_cap);
   const int lane = threadIdx.x & 63;
-  const int stride = gridDim.x * kBlock;
-  int q = blockIdx.x * kBlock + threadIdx.x - stride;
+  const int wave_id = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int nwaves = gridDim.x * (kBlock / 64);
+  const int nruns = (n + 63) / 64;
+  const int stream = wave_id < nruns ? ((nruns - 1 - wave_id) / nwaves + 1) * 64 : 0;
+  int cursor = 0;  // wave-uniform: next entry of the wave's stream
   GoldenState s{};
   bool active = false;
   uint32_t g = 0;
@@ -846,67 +860,80 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   double omega = 0., iw2 = 0., guard = 0.;
   float tau = 0.f;
   unsigned evals = 0;
-  bool exhausted = false;
-  // Each pass: lanes fetch probes and take golden steps until they need one
-  // evaluation of F (or run out of work); then the wave evaluates together.
-  // The outer exit is wave-uniform.
-  for (;;) {
-    while (!exhausted) {
-      if (!active) {
-        q += stride;
-        if (q >= n) { exhausted = true; break; }
-        g = from_list ? a.handoff_list[q] : a.trough_queue[q];
-        const int tl = g / kEventCap;
-        const Event &x = a.events[g];
-        const int kind = x.probe & 0xff;
-        if (kind != kTrough && kind != kDip) continue;
-        const bool resumed = !fp32 && a.fp32;
-        if (resumed && !(x.probe & kProbeNeeds64)) continue;
-        // Search only the neighbouring cells whose end values share the node's sign.
-        const NodeValues nv = node_values(a.events + static_cast<size_t>(tl) * kEventCap, g % kEventCap,
-                                          a.scan[tl].nev);
-        const bool left = x.c0 == x.c0 && signbit(nv.l) == signbit(nv.c);
-        const bool right = x.c2 == x.c2 && signbit(nv.r) == signbit(nv.c);
-        if (!left && !right) continue;
-        const TraceInfo ti = a.traces[tl + a.trace_begin];
-        const ModelInfo mi = a.models[ti.model];
-        rows = a.rows + mi.row0;
-        L = mi.L;
-        top = mi.top;
-        omega = ti.omega;
-        iw2 = 1. / (omega * omega);
-        guard = mi.cutoff_guard;
-        tau = ti.tau;
-        const double sign = signbit(nv.c) ? -1. : 1.;
-        if (resumed) {
-          golden_init(s, x.x, x.vx, sign, kind == kDip, x.sc);
-          s.it = (x.probe >> 16) & 0xff;
-          s.kmin = __int_as_float(x.pad);
-        } else {
-          golden_init(s, left ? x.c0 : x.c1, right ? x.c2 : x.c1, sign, kind == kDip, x.sc);
-        }
-        active = true;
-      }
-      double split, fsplit = 0.;
-      if (!golden_step(s, split, fsplit)) break;  // needs F at s.pending
-      if (fp32 && split == split) {
-        // An FP32 point of opposite sign can sit on the wrong side of a root
-        // that close (FP32 moves a step of F by ~1e-7): the FP64 pass repeats
-        // the search on the current bracket, so the split it keeps is an FP64
-        // sign change. Rare, so one atomic per lane.
-        Event &x = a.events[g];
-        x.x = s.lo;
-        x.vx = s.hi;
-        x.pad = __float_as_int(s.stage == 2 ? s.kmin : INFINITY);
-        x.probe |= kProbeNeeds64 | (s.it << 16);
-        const int slot = atomicAdd(a.handoff_count, 1);
-        if (slot < a.queue_cap) a.handoff_list[slot] = g;
-        active = false;
-        continue;
-      }
+  // A finished search: its split (NaN: no pair), or, in the FP32 pass, an
+  // opposite-sign point, which the FP64 pass must confirm. An FP32 point of
+  // opposite sign can sit on the wrong side of a root that close (FP32 moves
+  // a step of F by ~1e-7): the FP64 pass repeats the search on the current
+  // bracket, so the split it keeps is an FP64 sign change. Rare, so one
+  // atomic per lane.
+  auto finish = [&](double split, double fsplit) {
+    if (fp32 && split == split) {
+      Event &x = a.events[g];
+      x.x = s.lo;
+      x.vx = s.hi;
+      x.pad = __float_as_int(s.stage == 2 ? s.kmin : INFINITY);
+      x.probe |= kProbeNeeds64 | (s.it << 16);
+      const int slot = atomicAdd(a.handoff_count, 1);
+      if (slot < a.queue_cap) a.handoff_list[slot] = g;
+    } else {
       a.events[g].x = split;
       a.events[g].vx = fsplit;
-      active = false;
+    }
+    active = false;
+  };
+  // Each pass: lanes with an evaluated point take a step; idle lanes are
+  // handed the next entries (and take steps) until every lane needs one
+  // evaluation of F or the stream is done; then the wave evaluates together.
+  // Every loop condition is wave-uniform.
+  for (;;) {
+    if (active) {
+      double split, fsplit = 0.;
+      if (golden_step(s, split, fsplit)) finish(split, fsplit);
+    }
+    for (;;) {
+      const uint64_t idle = __ballot(!active);
+      if (!idle || cursor >= stream) break;
+      if (!active) {
+        const int v = cursor + __popcll(idle & lanes_below(lane));
+        const int q = (wave_id + (v >> 6) * nwaves) * 64 + (v & 63);
+        if (v < stream && q < n) {
+          g = from_list ? a.handoff_list[q] : a.trough_queue[q];
+          const int tl = g / kEventCap;
+          const Event &x = a.events[g];
+          const int kind = x.probe & 0xff;
+          const bool resumed = !fp32 && a.fp32;
+          if ((kind == kTrough || kind == kDip) && !(resumed && !(x.probe & kProbeNeeds64))) {
+            // Search only the neighbouring cells whose end values share the node's sign.
+            const NodeValues nv = node_values(a.events + static_cast<size_t>(tl) * kEventCap, g % kEventCap,
+                                              a.scan[tl].nev);
+            const bool left = x.c0 == x.c0 && signbit(nv.l) == signbit(nv.c);
+            const bool right = x.c2 == x.c2 && signbit(nv.r) == signbit(nv.c);
+            if (left || right) {
+              const TraceInfo ti = a.traces[tl + a.trace_begin];
+              const ModelInfo mi = a.models[ti.model];
+              rows = a.rows + mi.row0;
+              L = mi.L;
+              top = mi.top;
+              omega = ti.omega;
+              iw2 = 1. / (omega * omega);
+              guard = mi.cutoff_guard;
+              tau = ti.tau;
+              const double sign = signbit(nv.c) ? -1. : 1.;
+              if (resumed) {
+                golden_init(s, x.x, x.vx, sign, kind == kDip, x.sc);
+                s.it = (x.probe >> 16) & 0xff;
+                s.kmin = __int_as_float(x.pad);
+              } else {
+                golden_init(s, left ? x.c0 : x.c1, right ? x.c2 : x.c1, sign, kind == kDip, x.sc);
+              }
+              active = true;
+              double split, fsplit = 0.;
+              if (golden_step(s, split, fsplit)) finish(split, fsplit);
+            }
+          }
+        }
+      }
+      cursor += __popcll(idle);
     }
     if (!__ballot(active)) break;
     if (!active) continue;
@@ -914,10 +941,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     float ls;
     if constexpr (fp32) {
       // FP32 only while the interval is far wider than the FP32 spacing of k.
-      const double cs = snap32(omega, s.pending, s.lo, s.hi);
+      float k32;
+      const double cs = snap32(omega, s.pending, s.lo, s.hi, &k32);
       bool handed = !(cs < guard && s.hi - s.lo > 2e-6 * s.hi && cs > s.lo && cs < s.hi);
       if (!handed) {
-        const float f = secular_global<float>(rows, L, top, Wave, omega, iw2, float(omega / cs), &ls);
+        const float f = secular_global<float>(rows, L, top, Wave, omega, iw2, k32, &ls);
         handed = !(fabsf(f) >= tau);
         if (!handed) {
           const double sf = s.sign * double(f);
@@ -1311,77 +1339,151 @@ __device__ __forceinline__ double unscale(double f, const ScaleProduct &p, float
   return g == 0. && f != 0. ? copysign(4.9406564584124654e-324, f) : g;
 }
 
+// A guard root (beyond the reported modes) serves only the close-root tests
+// with its neighbours (trace flag, group-velocity CLOSE and window). When the
+// brackets alone decide both tests -- r_m - r_{m-1} <= close_rtol r_m is
+// certainly true or certainly false for any roots inside the two brackets,
+// with a margin far above rounding -- any point of its bracket gives the
+// same flags and windows as the refined root, so it is not refined (unless
+// every root is wanted, as the anchors of a continuation are).
+__device__ __forceinline__ bool close_decided(double lo1, double hi1, double lo2, double hi2, double rtol) {
+  const double far = (1. - rtol) * lo2 - hi1, near = (1. - rtol) * hi2 - lo1;
+  return far > 1e-9 * lo2 || near < -1e-9 * hi2;
+}
+__device__ __forceinline__ bool guard_decided(const PipeArgs &a, size_t q, int m, int count, const Bracket &br) {
+  const Bracket prev = a.brackets[q - 1];
+  if (!close_decided(prev.lo, prev.hi, br.lo, br.hi, a.close_rtol)) return false;
+  if (m + 1 >= count) return true;
+  const Bracket next = a.brackets[q + 1];
+  return close_decided(br.lo, br.hi, next.lo, next.hi, a.close_rtol);
+}
+
+// Refinement work order. A tile is kRefineTile consecutive traces of the
+// chunk (the next periods of one or a few models); its items are its (mode,
+// trace) pairs, mode-major, so that 64 consecutive items are one mode of 64
+// neighbouring traces: roots that sit in the same propagating/evanescent
+// regime of every layer, so the branches of a wave's evaluation agree. A
+// wave takes the tiles wave, wave + waves, ... one at a time and hands a
+// tile's items out in order to whichever lanes are idle (ballot + popcount
+// prefix). The rows of the tile's models are staged in the wave's LDS when
+// they fit (kRefineRows), else read from global memory.
+constexpr int kRefineTile = 64;
+template <int Wave>
+struct RefineRows {
+  static constexpr int value = Wave == 0 ? 64 : 32;  // Love: a smaller LDS share keeps 4 blocks per CU
+};
+
 // One instantiation per method and wave type (both fixed for a launch): the
 // Love instantiation is not held to the Rayleigh evaluation's registers.
 template <int Method, int Wave>
-__global__ __launch_bounds__(kBlock) void refine_kernel(PipeArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void refine_kernel(PipeArgs a) {
+  constexpr int kRows = RefineRows<Wave>::value;
   const int ntl = a.trace_end - a.trace_begin;
-  const int total = ntl * a.kmax;
-  const int stride = gridDim.x * kBlock;
-  int q = blockIdx.x * kBlock + threadIdx.x - stride;  // advanced before each fetch
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const int wave_id = blockIdx.x * (kBlock / 64) + wib;
+  const int nwaves = gridDim.x * (kBlock / 64);
+  const int ntiles = (ntl + kRefineTile - 1) / kRefineTile;
+  const int tile_items = a.kmax * kRefineTile;
   // The lane's state lives in LDS, so that the evaluation of F has the
   // registers to itself.
   __shared__ RefineState state[kBlock];
+  __shared__ PrepRow staged_rows[kBlock / 64][kRows];
   RefineState &s = state[threadIdx.x];
   s = RefineState{};
-  bool active = false;
   unsigned evals = 0;
-  const PrepRow *rows = nullptr;
-  int L = 0, top = 0;
-  double omega = 0.;
-  float ref = NAN;  // reference log2 scale of the unnormalized F; NaN: refine F itself
-  bool exhausted = false;
-  // Each pass: lanes fetch brackets and take Brent steps until they need one
-  // evaluation of F (or run out of work); then the wave evaluates together.
-  // The outer exit is wave-uniform.
-  for (;;) {
-    while (!exhausted) {
-      if (!active) {
-        q += stride;
-        if (q >= total) { exhausted = true; break; }
-        const int tl = q / a.kmax, m = q % a.kmax;
-        const int count = a.scan[tl].count;
-        const TraceInfo ti = a.traces[tl + a.trace_begin];
-        const ModelInfo mi = a.models[ti.model];
-        const Bracket br = m < count && m < mi.K ? a.brackets[q] : Bracket{NAN, NAN, 0., 0., NAN, NAN};
-        if (!(br.lo == br.lo)) { a.roots[q] = NAN; continue; }
-        if (br.flo == 0.) { a.roots[q] = br.lo; continue; }
-        if (br.fhi == 0.) { a.roots[q] = br.hi; continue; }
-        // Guard roots beyond the requested modes only feed the close-root test.
-        const double tol = m < mi.modes ? a.tol : fmax(a.tol, kGuardTol);
-        // With both end scales known, refine G = F 2^(ls - ref): the same
-        // zeros (a positive factor), and smooth where the normalized F of a
-        // deep evanescent stack is a +-1 step that defeats interpolation.
-        const bool scaled = fabsf(br.slo) < INFINITY && fabsf(br.shi) < INFINITY;
-        ref = scaled ? fmaxf(br.slo, br.shi) : NAN;
-        const double glo = scaled ? unscale(br.flo, double(br.slo) - double(ref)) : br.flo;
-        const double ghi = scaled ? unscale(br.fhi, double(br.shi) - double(ref)) : br.fhi;
-        refine_init(s, Method, br.lo, br.hi, glo, ghi, tol);
-        rows = a.rows + mi.row0;
-        L = mi.L;
-        top = mi.top;
-        omega = ti.omega;
-        active = true;
-      }
-      double root;
-      if (!refine_step(s, Method, root)) break;  // needs F(s.x)
-      a.roots[q] = root;
-      active = false;
+  for (int tile = wave_id; tile < ntiles; tile += nwaves) {
+    // The tile's rows: the span of the models of its traces that hold brackets.
+    const int t0 = tile * kRefineTile;
+    int lo = INT_MAX, hi = INT_MIN;
+    if (t0 + lane < ntl && a.scan[t0 + lane].count > 0) {
+      const ModelInfo &mi = a.models[a.traces[t0 + lane + a.trace_begin].model];
+      lo = mi.row0;
+      hi = mi.row0 + mi.L;
     }
-    if (!__ballot(active)) break;
-    if (!active) continue;
-    ++evals;
-    const double x = refine_point(s, Method);
-    __asm__ volatile("" ::: "memory");  // keep no copy of the state in registers across the evaluation
-    ScaleProduct sp;
-    double f = secular_rows(rows, L, top, Wave, omega, x, nullptr, &sp);
-    if (ref == ref) f = unscale(f, sp, ref);
-    __asm__ volatile("" ::: "memory");  // reload the state from LDS after the evaluation
-    if (!(f == f) || fabs(f) == INFINITY) {
-      a.roots[q] = NAN;
-      active = false;
-    } else {
-      refine_store(s, Method, f);
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = min(lo, __shfl_xor(lo, o, 64));
+      hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    // Wave-uniform. A tile with no brackets keeps lo = INT_MAX > hi = INT_MIN
+    // and stages nothing (hi - lo would overflow, so lo <= hi is tested first).
+    const bool staged = lo <= hi && hi - lo <= kRows;
+    wave_sync();  // the previous tile's evaluations have read the staged rows
+    if (staged && lane < hi - lo) staged_rows[wib][lane] = a.rows[lo + lane];
+    wave_sync();
+    bool active = false;
+    size_t q = 0;  // the lane's item: bracket / root index tl * kmax + m
+    int row0 = 0, L = 0, top = 0;
+    double omega = 0.;
+    float ref = NAN;  // reference log2 scale of the unnormalized F; NaN: refine F itself
+    int cursor = 0;   // wave-uniform: next item of the tile
+    // Each pass: lanes with an evaluated point take a step; idle lanes are
+    // handed the next items (and take steps) until every lane needs one
+    // evaluation of F or the tile is done; then the wave evaluates together.
+    // Every loop condition is wave-uniform.
+    for (;;) {
+      if (active) {
+        double root;
+        if (refine_step(s, Method, root)) {
+          a.roots[q] = root;
+          active = false;
+        }
+      }
+      for (;;) {
+        const uint64_t idle = __ballot(!active);
+        if (!idle || cursor >= tile_items) break;
+        if (!active) {
+          const int v = cursor + __popcll(idle & lanes_below(lane));
+          const int m = v / kRefineTile, tl = t0 + (v - m * kRefineTile);
+          if (v < tile_items && tl < ntl) {
+            q = static_cast<size_t>(tl) * a.kmax + m;
+            const int count = a.scan[tl].count;
+            const TraceInfo ti = a.traces[tl + a.trace_begin];
+            const ModelInfo mi = a.models[ti.model];
+            const Bracket br = m < count && m < mi.K ? a.brackets[q] : Bracket{NAN, NAN, 0., 0., NAN, NAN};
+            if (!(br.lo == br.lo)) a.roots[q] = NAN;
+            else if (br.flo == 0.) a.roots[q] = br.lo;
+            else if (br.fhi == 0.) a.roots[q] = br.hi;
+            else if (m >= mi.modes && !a.all_roots && guard_decided(a, q, m, min(count, mi.K), br))
+              a.roots[q] = bracket_root(br.lo, br.hi, br.flo, br.fhi);
+            else {
+              // Guard roots beyond the requested modes only feed the close-root test.
+              const double tol = m < mi.modes ? a.tol : fmax(a.tol, kGuardTol);
+              // With both end scales known, refine G = F 2^(ls - ref): the same
+              // zeros (a positive factor), and smooth where the normalized F of a
+              // deep evanescent stack is a +-1 step that defeats interpolation.
+              const bool scaled = fabsf(br.slo) < INFINITY && fabsf(br.shi) < INFINITY;
+              ref = scaled ? fmaxf(br.slo, br.shi) : NAN;
+              const double glo = scaled ? unscale(br.flo, double(br.slo) - double(ref)) : br.flo;
+              const double ghi = scaled ? unscale(br.fhi, double(br.shi) - double(ref)) : br.fhi;
+              refine_init(s, Method, br.lo, br.hi, glo, ghi, tol);
+              row0 = mi.row0;
+              L = mi.L;
+              top = mi.top;
+              omega = ti.omega;
+              double root;
+              if (refine_step(s, Method, root)) a.roots[q] = root;
+              else active = true;
+            }
+          }
+        }
+        cursor += __popcll(idle);
+      }
+      if (!__ballot(active)) break;
+      if (!active) continue;
+      ++evals;
+      const double x = refine_point(s, Method);
+      __asm__ volatile("" ::: "memory");  // keep no copy of the state in registers across the evaluation
+      ScaleProduct sp;
+      const PrepRow *rows = staged ? &staged_rows[wib][row0 - lo] : a.rows + row0;
+      double f = secular_rows(rows, L, top, Wave, omega, x, nullptr, &sp);
+      if (ref == ref) f = unscale(f, sp, ref);
+      __asm__ volatile("" ::: "memory");  // reload the state from LDS after the evaluation
+      if (!(f == f) || fabs(f) == INFINITY) {
+        a.roots[q] = NAN;
+        active = false;
+      } else {
+        refine_store(s, Method, f);
+      }
     }
   }
   if (a.stats) wave_count(a.stats + 6, evals);

@@ -137,6 +137,9 @@ MDR_HD double fast_rcp(double x) {
 template <class T> MDR_HD T tiny_scale();
 template <> MDR_HD float tiny_scale<float>() { return 1.0e-30f; }
 template <> MDR_HD double tiny_scale<double>() { return 1.0e-40; }
+#if !defined(__HIPCC__)
+template <> MDR_HD long double tiny_scale<long double>() { return 1.0e-40L; }  // host-only checks
+#endif
 
 // Max-abs rescale: multiply by ~1/max|e_i| so the state stays O(1). The scale
 // is positive, so neither the sign nor the zeros of F change; it is continuous
@@ -146,13 +149,12 @@ MDR_HD T rescale(T *e) {
   T m = T(0);
 #pragma unroll
   for (int i = 0; i < N; ++i) m = fmax(m, m_fabs(e[i]));
-  if (m > tiny_scale<T>()) {
-    const T inv = fast_rcp(m);
+  // A vanishing state is left unscaled (scale 1); one select, not one per entry.
+  const bool big = m > tiny_scale<T>();
+  const T inv = big ? fast_rcp(m) : T(1);
 #pragma unroll
-    for (int i = 0; i < N; ++i) e[i] *= inv;
-    return m;
-  }
-  return T(1);
+  for (int i = 0; i < N; ++i) e[i] *= inv;
+  return big ? m : T(1);
 }
 
 // log2 of a (positive) normalization scale, as float. A trapped-mode
@@ -216,31 +218,30 @@ MDR_HD void root_pair(T r2, T *r, T *ir) {
 
 template <class T>
 MDR_HD Wave<T> wave_factor(T k, T kx, T h) {
-  Wave<T> o;
+  // At k = kx, r^2 = 0 is raised to a tiny positive value: r stays finite,
+  // p rounds e^-p to exactly 1, so c = 1, s = 0, x = 0 and the attenuation
+  // is 1 -- the r -> 0 limit -- and only w needs its limit h (below).
   T r, ir;
-  root_pair((k + kx) * m_fabs(k - kx), &r, &ir);
+  root_pair(fmax((k + kx) * m_fabs(k - kx), tiny_scale<T>()), &r, &ir);
   const T p = r * h;
-  if (k < kx) {  // propagating: cos p, sin p / r, -r sin p
-    T s, c;
+  // The branches produce only (cosine-like, sine-like, attenuation); the
+  // shared tail forms w and x, so a divergent wave merges three values.
+  T c, s, att;
+  if (k < kx) {  // propagating: cos p, sin p
     m_sincos(p, &s, &c);
-    o.c = c;
-    o.w = s * ir;
-    o.x = -r * s;
-    o.att = T(1);
-  } else if (k == kx) {  // c equals the body-wave speed: the r -> 0 limit
-    o.c = T(1);
-    o.w = h;
-    o.x = T(0);
-    o.att = T(1);
-  } else {  // evanescent, scaled by e^-p: cosh p, sinh p / r, r sinh p
+    att = T(1);
+  } else {  // evanescent, scaled by e^-p: cosh p, sinh p
     const T e = m_exp(-p);
     const T f = e * e;
-    o.c = (T(1) + f) * T(0.5);
-    const T s = (T(1) - f) * T(0.5);
-    o.w = s * ir;
-    o.x = r * s;
-    o.att = e;
+    c = (T(1) + f) * T(0.5);
+    s = (T(1) - f) * T(0.5);
+    att = e;
   }
+  Wave<T> o;
+  o.c = c;
+  o.w = k == kx ? h : s * ir;  // sin p / r or sinh p / r; the r -> 0 limit h at k = kx
+  o.x = k < kx ? -r * s : r * s;
+  o.att = att;
   return o;
 }
 

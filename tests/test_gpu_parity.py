@@ -318,6 +318,23 @@ def test_halfspace_rayleigh_and_empty_families(md):
     assert (empty["search_level"] == -4).all() and (empty["root_count"] == 0).all()
 
 
+@pytest.mark.parametrize("wave", [0, 1], ids=["rayleigh", "love"])
+def test_refinement_tiles_without_brackets(md, wave):
+    """The refinement takes traces in tiles of 64 and stages the rows of a
+    tile's models in LDS; tiles where no trace holds a bracket (here the first
+    300 traces: three models with empty windows) stage nothing, and the models
+    after them equal the oracle."""
+    from modisp_amd import workloads
+    wl = workloads.precision_batch(B=8, seed=107)
+    win = np.array(workloads.scan_window(wl.vs), dtype=np.float64)
+    win[:3] = win[:3, ::-1]
+    got = gpu_solve(md, *(getattr(wl, k) for k in MODEL), wl.periods, win, 5, wave)
+    ref = oracle.solve(*(getattr(wl, k) for k in MODEL), wl.periods, win, 5, wave, tol=1e-12, group=2)
+    assert (got["root_count"][:3] == 0).all() and (got["search_level"][:3] == -4).all()
+    assert (got["root_count"][3:] > 0).all()
+    assert_parity(got, ref, label=f"wave {wave}")
+
+
 # --- full-size, size-independent properties ------------------------------------------------------
 @pytest.mark.parametrize("wave", [0, 1], ids=["rayleigh", "love"])
 def test_full_precision_workload_fp32_and_fp64_paths(md, wave):
