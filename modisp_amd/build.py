@@ -10,7 +10,8 @@ ROOT = HERE.parent
 LIBRARY = HERE / "libmodisp_rocm.so"
 SOURCES = [HERE / "csrc" / "kernels.hip", HERE / "csrc" / "adjoint.hip", HERE / "csrc" / "solver.cpp",
            HERE / "csrc" / "bounds.cpp", HERE / "csrc" / "results.cpp"]
-HEADERS = [HERE / "csrc" / "secular.hpp", HERE / "csrc" / "pipeline.hpp", ROOT / "include" / "modisp_rocm.h"]
+HEADERS = [HERE / "csrc" / "secular.hpp", HERE / "csrc" / "pipeline.hpp", HERE / "csrc" / "adjoint_root.hpp",
+           ROOT / "include" / "modisp_rocm.h"]
 
 
 def hipcc() -> str:
