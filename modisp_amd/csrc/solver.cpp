@@ -14,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/modisp_rocm.h"
@@ -60,6 +61,94 @@ struct DevBuf {
   template <class T>
   T *as() const { return static_cast<T *>(p); }
 };
+
+// Pinned host memory that results from host-array calls pass through:
+// device-to-host copies into pinned memory run at the link's rate, where
+// copies into the caller's pageable arrays are staged by the runtime at a
+// fraction of it; the host then moves the bytes on into the caller's arrays
+// with several threads (which also share the cost of first touching freshly
+// allocated caller pages). Allocated once per solver and grown on demand.
+struct HostStage {
+  void *p = nullptr;
+  size_t bytes = 0;
+  bool ensure(size_t n) {
+    if (n <= bytes) return true;
+    release();
+    if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return false;
+    }
+    bytes = n;
+    return true;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// One result array of a host-array call: `bytes` from device `src` to host `dst`.
+struct HostCopy {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+
+// Copies every part on the solver's stream through pinned staging (or, if
+// pinned memory cannot be had, straight into the caller's memory) and
+// synchronizes the stream. The parts are taken as one byte stream in windows
+// of at most kStageBytes: each window is copied to the staging, then spread
+// into the caller's arrays by up to eight host threads.
+constexpr size_t kStageBytes = size_t(256) << 20;
+int copy_to_host(hipStream_t st, HostStage &stage, const std::vector<HostCopy> &parts) {
+  size_t payload = 0;
+  for (const HostCopy &c : parts) payload += c.bytes;
+  if (payload == 0) return hipStreamSynchronize(st) == hipSuccess ? 0 : fail(1, "stream synchronization failed");
+  if (!stage.ensure(std::min(payload, kStageBytes))) {
+    for (const HostCopy &c : parts)
+      if (c.bytes) MD_HIP(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, st));
+    MD_HIP(hipStreamSynchronize(st));
+    return 0;
+  }
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  for (size_t w0 = 0; w0 < payload; w0 += kStageBytes) {
+    const size_t w1 = std::min(payload, w0 + kStageBytes);
+    // Device-to-pinned copies of the window's pieces (stream offset - w0 in the staging).
+    size_t base = 0;
+    for (const HostCopy &c : parts) {
+      const size_t a = std::max(w0, base), b = std::min(w1, base + c.bytes);
+      if (a < b)
+        MD_HIP(hipMemcpyAsync(static_cast<char *>(stage.p) + (a - w0), static_cast<const char *>(c.src) + (a - base),
+                              b - a, hipMemcpyDeviceToHost, st));
+      base += c.bytes;
+    }
+    MD_HIP(hipStreamSynchronize(st));
+    auto spread = [&](size_t lo, size_t hi) {  // stream bytes [lo, hi) of this window
+      size_t at = 0;
+      for (const HostCopy &c : parts) {
+        const size_t a = std::max(lo, at), b = std::min(hi, at + c.bytes);
+        if (a < b)
+          std::memcpy(static_cast<char *>(c.dst) + (a - at), static_cast<const char *>(stage.p) + (a - w0), b - a);
+        at += c.bytes;
+      }
+    };
+    const size_t n = w1 - w0;
+    const int nt = static_cast<int>(std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, n >> 23)));
+    if (nt <= 1) {
+      spread(w0, w1);
+      continue;
+    }
+    std::vector<std::thread> pool;
+    const size_t step = (n + nt - 1) / nt;
+    for (int t = 1; t < nt; ++t)
+      pool.emplace_back(spread, std::min(w1, w0 + t * step), std::min(w1, w0 + (t + 1) * step));
+    spread(w0, std::min(w1, w0 + step));
+    for (std::thread &th : pool) th.join();
+  }
+  return 0;
+}
 
 constexpr int kStatCount = mdr::kStats;  // see md_rocm_last_stats
 
@@ -135,6 +224,7 @@ struct md_rocm_solver {
   int prep_rows = 0;               // largest row count among them
   DevBuf table, ntab;              // per-model scan tables (adaptive grid)
   DevBuf invalid;                  // models of the last device ragged run with inconsistent offsets
+  HostStage host_stage;            // pinned staging of host-array results
   bool invalid_live = false;       // `invalid` belongs to the last run
 
   ~md_rocm_solver() {
@@ -144,6 +234,7 @@ struct md_rocm_solver {
                       &in_h, &in_vp, &in_vs, &in_rho, &in_per, &in_bounds, &in_offsets, &out_v, &out_flags, &out_levels,
                       &out_counts, &out_group, &out_gflags, &out_v32, &table, &ntab, &invalid})
       b->release();
+    host_stage.release();
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (hipEvent_t e : marks) (void)hipEventDestroy(e);
@@ -1016,20 +1107,22 @@ int md_rocm_run(md_rocm_solver *s, const double *thickness, const double *vp, co
                                         want_group ? s->out_gflags.as<uint32_t>() : nullptr)) {
     return e;
   }
+  std::vector<HostCopy> parts;
   if (velocity_is_f32) {  // narrowed on the device: half the bytes cross to the host
     if (int e = s->out_v32.ensure(ncell * sizeof(float))) return e;
     MD_HIP(mdr::launch_narrow(s->out_v.as<double>(), s->out_v32.as<float>(), ncell, st));
-    MD_HIP(hipMemcpyAsync(velocity, s->out_v32.p, ncell * sizeof(float), hipMemcpyDeviceToHost, st));
+    parts.push_back({velocity, s->out_v32.p, ncell * sizeof(float)});
   } else {
-    MD_HIP(hipMemcpyAsync(velocity, s->out_v.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
+    parts.push_back({velocity, s->out_v.p, ncell * sizeof(double)});
   }
-  MD_HIP(hipMemcpyAsync(flags, s->out_flags.p, ntr * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  MD_HIP(hipMemcpyAsync(levels, s->out_levels.p, ntr * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  MD_HIP(hipMemcpyAsync(counts, s->out_counts.p, ntr * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  parts.push_back({flags, s->out_flags.p, ntr * sizeof(uint32_t)});
+  parts.push_back({levels, s->out_levels.p, ntr * sizeof(int32_t)});
+  parts.push_back({counts, s->out_counts.p, ntr * sizeof(int32_t)});
   if (want_group) {
-    MD_HIP(hipMemcpyAsync(group, s->out_group.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
-    MD_HIP(hipMemcpyAsync(group_flags, s->out_gflags.p, ncell * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    parts.push_back({group, s->out_group.p, ncell * sizeof(double)});
+    parts.push_back({group_flags, s->out_gflags.p, ncell * sizeof(uint32_t)});
   }
+  if (int e = copy_to_host(st, s->host_stage, parts)) return e;
   if (int e = md_rocm_synchronize(s)) return e;
   if (prediction_flags) {
     if (s->cfg.continuation) {
@@ -1146,22 +1239,17 @@ int md_rocm_run_ragged(md_rocm_solver *s, const double *thickness, const double 
     return e;
   }
   MD_HIP(hipEventRecord(s->ev_end, st));
-  MD_HIP(hipMemcpyAsync(velocity, s->out_v.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
-  MD_HIP(hipMemcpyAsync(flags, s->out_flags.p, static_cast<size_t>(ntr) * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                        st));
-  MD_HIP(hipMemcpyAsync(levels, s->out_levels.p, static_cast<size_t>(ntr) * sizeof(int32_t), hipMemcpyDeviceToHost,
-                        st));
-  MD_HIP(hipMemcpyAsync(counts, s->out_counts.p, static_cast<size_t>(ntr) * sizeof(int32_t), hipMemcpyDeviceToHost,
-                        st));
+  std::vector<HostCopy> parts = {{velocity, s->out_v.p, ncell * sizeof(double)},
+                                 {flags, s->out_flags.p, static_cast<size_t>(ntr) * sizeof(uint32_t)},
+                                 {levels, s->out_levels.p, static_cast<size_t>(ntr) * sizeof(int32_t)},
+                                 {counts, s->out_counts.p, static_cast<size_t>(ntr) * sizeof(int32_t)}};
   if (want_group) {
-    MD_HIP(hipMemcpyAsync(group, s->out_group.p, ncell * sizeof(double), hipMemcpyDeviceToHost, st));
-    MD_HIP(hipMemcpyAsync(group_flags, s->out_gflags.p, ncell * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    parts.push_back({group, s->out_group.p, ncell * sizeof(double)});
+    parts.push_back({group_flags, s->out_gflags.p, ncell * sizeof(uint32_t)});
   }
-  if (prediction_flags) {
-    if (s->cfg.continuation)
-      MD_HIP(hipMemcpyAsync(prediction_flags, s->c_pred.p, static_cast<size_t>(ntr) * sizeof(uint32_t),
-                            hipMemcpyDeviceToHost, st));
-  }
+  if (prediction_flags && s->cfg.continuation)
+    parts.push_back({prediction_flags, s->c_pred.p, static_cast<size_t>(ntr) * sizeof(uint32_t)});
+  if (int e = copy_to_host(st, s->host_stage, parts)) return e;
   if (int e = md_rocm_synchronize(s)) return e;
   if (prediction_flags) {
     for (int m = 0; m < B; ++m)
