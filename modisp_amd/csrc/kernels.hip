@@ -144,6 +144,19 @@ __device__ __forceinline__ void wave_count(unsigned long long *stat, unsigned n)
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(stat, v);
 }
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
+// The value held by lane - 1 (from_left) or lane + 1 (from_right), modulo 64:
+// one DPP wave rotate each (wave_ror:1 / wave_rol:1), no LDS round trip.
+// Every lane of the wave must be active.
+__device__ __forceinline__ int from_left(int x) { return __builtin_amdgcn_mov_dpp(x, 0x13C, 0xF, 0xF, false); }
+__device__ __forceinline__ int from_right(int x) { return __builtin_amdgcn_mov_dpp(x, 0x134, 0xF, 0xF, false); }
+__device__ __forceinline__ float from_left(float x) { return __int_as_float(from_left(__float_as_int(x))); }
+__device__ __forceinline__ float from_right(float x) { return __int_as_float(from_right(__float_as_int(x))); }
+__device__ __forceinline__ double from_left(double x) {
+  return __hiloint2double(from_left(__double2hiint(x)), from_left(__double2loint(x)));
+}
+__device__ __forceinline__ double from_right(double x) {
+  return __hiloint2double(from_right(__double2hiint(x)), from_right(__double2loint(x)));
+}
 __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -327,9 +340,9 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
       f = float(F);
       u = fp32 && (!(fabsf(f) >= tau) || c >= hi_guard || end || c < mi.stable_low);
     }
-    const float fl = __shfl(f, (lane + 63) & 63, 64), fr = __shfl(f, (lane + 1) & 63, 64);
-    const float sl = __shfl(sc, (lane + 63) & 63, 64), sr = __shfl(sc, (lane + 1) & 63, 64);
-    const double cl = __shfl(c, (lane + 63) & 63, 64), cr = __shfl(c, (lane + 1) & 63, 64);
+    const float fl = from_left(f), fr = from_right(f);
+    const float sl = from_left(sc), sr = from_right(sc);
+    const double cl = from_left(c), cr = from_right(c);
     // Nodes closer than 1e-6 (relative) to a neighbour -- the quadratic
     // spacing after a layer speed packs them there when many modes crowd
     // above it -- are below what FP32 resolves (its k steps by 6e-8; snapped
@@ -357,7 +370,7 @@ __device__ void scan_wave(const PipeArgs &a, int t, WaveScratch<T, R> &sh, int l
     const bool change = active && has_right && !u && !ur && lane <= 62 && signbit(f) != signbit(fr);
     const uint64_t cm = __ballot(change);
     const bool change_l = lane >= 1 && ((cm >> (lane - 1)) & 1);
-    const float f2 = __shfl(fr, (lane + 1) & 63, 64);  // value two nodes to the right
+    const float f2 = from_right(fr);  // value two nodes to the right
     bool verify = false;
     if (fp32 && change) {
       const float a1 = fabsf(f), a2 = fabsf(fr), amin = fminf(a1, a2), amax = fmaxf(a1, a2);
