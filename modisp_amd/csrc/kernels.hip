@@ -23,23 +23,27 @@
 //   recheck_kernel     FP64 value of every uncertain node (dense lanes).
 //   unc_probe_kernel   trough test at the rechecked nodes.
 //   trough_kernel      golden-section probes of the candidates, one search per
-//                      lane as a state machine: an FP32 pass, and an FP64 pass
-//                      over the searches it handed over (or every probe of the
-//                      FP64 path); a point of opposite sign splits the cell.
+//                      lane as a state machine, idle lanes fed the next probes
+//                      of the wave's runs in order: an FP32 pass, and an FP64
+//                      pass over the searches it handed over (or every probe of
+//                      the FP64 path); a point of opposite sign splits the cell.
 //   assemble_kernel    one lane per trace: ordered brackets and the root count.
 //   love_count_kernel  Love: the exact phase count verifies each trace;
 //   love_phase_kernel  a disagreeing trace is re-bracketed by phase bisection.
 //   refine_kernel      one bracket per lane as a state machine: FP64
 //                      Brent-Dekker (default), Chandrupatla or the legacy
-//                      regula falsi (reference cpp/common/refinement.hpp:11-65).
+//                      regula falsi (reference cpp/common/refinement.hpp:11-65);
+//                      tiles of 64 traces taken mode-major, idle lanes fed in
+//                      order, the tile's rows staged in LDS; guard roots whose
+//                      close-root tests their brackets decide are not refined.
 //   finalize_kernel    outputs in the reference layout, trace flags.
 //   group_kernel       mode-major lanes: analytic implicit group velocity
 //                      U = -dF/dk / dF/dw by forward-mode dual numbers.
 //   slope/anchor_flag/continuation/gather kernels: anchor continuation.
 //
 // Work counts live in device memory and the consumer kernels are persistent
-// grid-stride kernels, so the sequence runs back to back on one stream with no
-// host synchronisation and can be captured in a hipGraph.
+// kernels that read them, so the sequence runs back to back on one stream with
+// no host synchronisation and can be captured in a hipGraph.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "pipeline.hpp"
