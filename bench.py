@@ -452,6 +452,37 @@ def uneven_leg(torch, modisp_amd, workloads, models, steps):
     return out
 
 
+def uneven_profile(torch, modisp_amd, workloads, models, steps):
+    """Profiling mode: the uneven ragged batch resident on the device, one
+    warm-up run and `steps` eager runs with the default launch shape; per-run
+    stage milliseconds and the work counters of the last run."""
+    import numpy as np
+    wl = workloads.uneven_batch(B=models)
+    packed = workloads.pack_ragged(wl)
+    geom, cells = modisp_amd.ragged_geometry(packed["row_offsets"], packed["period_offsets"], packed["modes"])
+    dev = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    bufs = {k: t(packed[k]) for k in ("thickness", "vp", "vs", "rho", "row_offsets", "periods", "period_offsets",
+                                      "modes", "bounds")}
+    bufs["cell_offsets"] = t(cells)
+    bufs.update(velocity=torch.empty(geom["ncells"], dtype=torch.float64, device=dev),
+                flags=torch.empty(geom["ntraces"], dtype=torch.int32, device=dev),
+                levels=torch.empty(geom["ntraces"], dtype=torch.int32, device=dev),
+                counts=torch.empty(geom["ntraces"], dtype=torch.int32, device=dev))
+    ptr = {k: v.data_ptr() for k, v in bufs.items()}
+    runs = []
+    with modisp_amd.QeSolver(geom["B"], geom["max_rows"], 1, 1, scan="float32") as s:
+        s.run_ragged_device_batch(geom, ptr, 1e-12, 0)
+        s.synchronize()
+        for _ in range(steps):
+            s.run_ragged_device_batch(geom, ptr, 1e-12, 0)
+            s.synchronize()
+            runs.append(dict(zip(s.STAGES, map(float, s.last_timing()))))
+        stats = s.last_stats()
+    return {"workload": f"uneven: {geom['B']} models, {geom['ntraces']} traces, Rayleigh, FP32 scan",
+            "runs": runs, "work": stats, "roots": int(bufs["velocity"].isfinite().sum().item())}
+
+
 def sweep_leg(torch, modisp_amd, workloads, steps):
     """The inversion-scale sweep (262144 models of 30 layers + half-space,
     200 periods 0.5-100 s, Rayleigh and Love, 4 modes, group velocity):
@@ -549,6 +580,9 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the inversion-scale sweep leg")
     ap.add_argument("--no-graph", action="store_true", help="skip the hipGraph replay leg")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end Solver leg (host arrays)")
+    ap.add_argument("--profile-uneven", action="store_true",
+                    help="profiling mode: only `steps` eager runs of the uneven ragged batch resident on the device "
+                         "(stage times and work counters as one JSON line), for a kernel trace of that workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every trace is independent, "
@@ -565,6 +599,9 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     modisp_amd.set_device(0)
+    if args.profile_uneven:
+        print(json.dumps(uneven_profile(torch, modisp_amd, workloads, args.uneven_models, args.steps)))
+        return
     wl = workloads.precision_batch(B=args.models)
     window = workloads.scan_window(wl.vs)
     L = wl.thickness.shape[1]
