@@ -83,9 +83,15 @@ void all_masks(const V *v, const uint32_t *flags, const int32_t *counts, const d
     return;
   }
   std::vector<std::thread> pool;
-  for (int i = 0; i < nthreads; ++i)
-    pool.emplace_back(work, static_cast<int>(static_cast<long long>(B) * i / nthreads),
-                      static_cast<int>(static_cast<long long>(B) * (i + 1) / nthreads));
+  for (int i = 0; i < nthreads; ++i) {
+    const int b0 = static_cast<int>(static_cast<long long>(B) * i / nthreads);
+    const int b1 = static_cast<int>(static_cast<long long>(B) * (i + 1) / nthreads);
+    try {
+      pool.emplace_back(work, b0, b1);
+    } catch (...) {  // a thread that cannot start: its models are done here
+      work(b0, b1);
+    }
+  }
   for (std::thread &th : pool) th.join();
 }
 

@@ -142,9 +142,16 @@ int copy_to_host(hipStream_t st, HostStage &stage, const std::vector<HostCopy> &
     }
     std::vector<std::thread> pool;
     const size_t step = (n + nt - 1) / nt;
-    for (int t = 1; t < nt; ++t)
-      pool.emplace_back(spread, std::min(w1, w0 + t * step), std::min(w1, w0 + (t + 1) * step));
-    spread(w0, std::min(w1, w0 + step));
+    size_t done = w0 + step;  // the calling thread takes [w0, w0 + step); a thread that cannot start is done here
+    for (int t = 1; t < nt; ++t) {
+      const size_t a = std::min(w1, w0 + t * step), b = std::min(w1, w0 + (t + 1) * step);
+      try {
+        pool.emplace_back(spread, a, b);
+      } catch (...) {
+        spread(a, b);
+      }
+    }
+    spread(w0, std::min(w1, done));
     for (std::thread &th : pool) th.join();
   }
   return 0;
