@@ -84,8 +84,14 @@ extern "C" int md_rocm_bounds(const double *thickness, const double *vp, const d
     return 0;
   }
   std::vector<std::thread> pool;
-  for (int t = 0; t < nthreads; ++t)
-    pool.emplace_back(window, static_cast<int>(1LL * B * t / nthreads), static_cast<int>(1LL * B * (t + 1) / nthreads));
+  for (int t = 0; t < nthreads; ++t) {
+    const int b0 = static_cast<int>(1LL * B * t / nthreads), b1 = static_cast<int>(1LL * B * (t + 1) / nthreads);
+    try {
+      pool.emplace_back(window, b0, b1);
+    } catch (...) {  // a thread that cannot start: its models are done here
+      window(b0, b1);
+    }
+  }
   for (std::thread &th : pool) th.join();
   return 0;
 }
