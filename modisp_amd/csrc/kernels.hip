@@ -28,6 +28,8 @@
 //                      pass over the searches it handed over (or every probe of
 //                      the FP64 path); a point of opposite sign splits the cell.
 //   assemble_kernel    one lane per trace: ordered brackets and the root count.
+//   love_range_kernel  Love: the exact phase count of each trace's window (a
+//                      trace with one takes no pair probes);
 //   love_count_kernel  Love: the exact phase count verifies each trace;
 //   love_phase_kernel  a disagreeing trace is re-bracketed by phase bisection.
 //   refine_kernel      one bracket per lane as a state machine: FP64
@@ -665,7 +667,13 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(PipeArgs a) {
   uint64_t um = 0ull, pm = 0ull;
   if (live) {
     const TraceScan ts = a.scan[tl];
-    if (ts.status == 0 && !(ts.flags & kTraceOverflow)) { um = ts.umask; pm = ts.pmask; }
+    if (ts.status == 0 && !(ts.flags & kTraceOverflow)) {
+      um = ts.umask;
+      // A Love trace whose phase count is known needs no pair probe: the
+      // count verifies its brackets and re-brackets it by phase where a pair
+      // hides (love_count_kernel, love_phase_kernel).
+      pm = (ts.flags & kScanPhaseReady) ? 0ull : ts.pmask;
+    }
   }
   const int nu = __popcll(um), np = __popcll(pm);
   const int iu = wave_incl_scan(nu, lane), ip = wave_incl_scan(np, lane);
@@ -745,7 +753,8 @@ __global__ __launch_bounds__(kBlock) void unc_probe_kernel(PipeArgs a) {
       const double low = a.fp32 ? a.models[a.traces[tl + a.trace_begin].model].stable_low : 0.;
       const int kind = x.probe & 0xff;
       const bool has_l = x.c0 == x.c0, has_r = x.c2 == x.c2;
-      if (kind != kTrough && (has_l || has_r) && !(has_l && x.c0 < low) && !(!has_l && x.c1 < low)) {
+      const bool phase_ready = (a.scan[tl].flags & kScanPhaseReady) != 0;  // no pair probes (compact_kernel)
+      if (!phase_ready && kind != kTrough && (has_l || has_r) && !(has_l && x.c0 < low) && !(!has_l && x.c1 < low)) {
         const NodeValues v = node_values(ev, e, a.scan[tl].nev);
         // Both neighbours, or the one neighbour of a window end.
         const double al = has_l ? fabs(v.l) : INFINITY, ar = has_r ? fabs(v.r) : INFINITY;
@@ -1104,9 +1113,11 @@ __device__ inline int love_roots_between(double plo, double pc) {
   return n > 0. ? int(n) : 0;
 }
 
-// One lane per Love trace: compares the assembled brackets with the exact
-// phase count and queues the trace for a phase search when they disagree.
-__global__ __launch_bounds__(kBlock) void love_count_kernel(PipeArgs a) {
+// One lane per Love trace, right after the scan (and the FP64 rescans): the
+// exact phase count of the trace's window, where the phase exists at both
+// ends. Such a trace is verified by the count after assembly, so it takes no
+// pair probes.
+__global__ __launch_bounds__(kBlock) void love_range_kernel(PipeArgs a) {
   const int tl = blockIdx.x * kBlock + threadIdx.x;
   const int t = tl + a.trace_begin;
   if (t >= a.trace_end) return;
@@ -1117,11 +1128,29 @@ __global__ __launch_bounds__(kBlock) void love_count_kernel(PipeArgs a) {
   const PrepRow *rows = a.rows + mi.row0;
   const double pl = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.lo, mi.vs_half);
   const double ph = love_phase_dev(rows, mi.L, mi.top, ti.omega, mi.hi, mi.vs_half);
-  if (!(pl == pl) || !(ph == ph) || ph > pl) {
+  if (!(pl == pl) || !(ph == ph) || ph > pl) return;
+  ts.exact = love_roots_between(pl, ph);
+  ts.phase_lo = pl;
+  ts.flags |= kScanPhaseReady;
+}
+
+// One lane per Love trace: compares the assembled brackets with the exact
+// phase count and queues the trace for a phase search when they disagree.
+__global__ __launch_bounds__(kBlock) void love_count_kernel(PipeArgs a) {
+  const int tl = blockIdx.x * kBlock + threadIdx.x;
+  const int t = tl + a.trace_begin;
+  if (t >= a.trace_end) return;
+  TraceScan &ts = a.scan[tl];
+  if (ts.status || (ts.flags & kTraceOverflow)) return;
+  if (!(ts.flags & kScanPhaseReady)) {  // no phase at a window end (love_range_kernel)
     ts.flags |= kFlagLoveFallback;
     return;
   }
-  const int exact = love_roots_between(pl, ph), K = mi.K;
+  const TraceInfo ti = a.traces[t];
+  const ModelInfo mi = a.models[ti.model];
+  const PrepRow *rows = a.rows + mi.row0;
+  const double pl = ts.phase_lo;
+  const int exact = ts.exact, K = mi.K;
   const int want = exact < K ? exact : K;
   bool ok = ts.count == want;
   if (ok && want == K && exact > K) {
@@ -2185,6 +2214,8 @@ hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bo
     else
       hipLaunchKernelGGL((rescan_kernel<kMaxRows, 1>), dim3(pb), dim3(64), 0, s, pa);
   }
+  if (pa.wave == 1 && pa.love_phase)
+    hipLaunchKernelGGL(love_range_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   mark(2);
   hipLaunchKernelGGL(compact_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, pa);
   if (pa.wave == 1) hipLaunchKernelGGL(recheck_kernel<1>, dim3(pb), dim3(kBlock), 0, s, pa);

@@ -50,6 +50,7 @@ constexpr unsigned kGroupMissing = 1u, kGroupPolish = 2u, kGroupUnstable = 4u, k
 constexpr int kLevelShapeHigh = 8, kLevelShapeBalanced = 9, kLevelShapeFast = 10, kLevelAdaptive = 3,
               kLevelObservationBalanced = 6, kLevelObservationFast = 7, kLevelLovePhase = 11;
 constexpr unsigned kScanLovePhase = 1u << 30;  // TraceScan: Love roots follow the exact phase count
+constexpr unsigned kScanPhaseReady = 1u << 29;  // TraceScan: the Love phase count of the window is known (exact, phase_lo)
 
 // One prepared layer row: reciprocals so a trace's constants are products.
 struct PrepRow {
@@ -120,6 +121,8 @@ struct TraceScan {
   int nodes;    // secular evaluations of the trace's scan (lane slots, batch overlaps included)
   uint64_t umask, pmask;  // events (bit e) that are FP32-uncertain / probed for a hidden pair
   unsigned rejected;      // reported modes (bit m) whose |F| failed the f_accept threshold
+  int exact;              // Love with kScanPhaseReady: roots in the window by the phase count
+  double phase_lo;        // Love with kScanPhaseReady: the unwrapped phase at the window bottom
 };
 
 // A root bracket with F at its ends and the summed log2 normalization scales
