@@ -1037,9 +1037,26 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(PipeArgs a) {
       emit(lo, x.x, flo, x.vx);
       if (nb < K) emit(x.x, hi, x.vx, fhi);
     };
-    for (int e = 0; e < ts.nev && nb < K; ++e) {
-      const Event x = ev[e];
-      const NodeValues v = node_values(ev, e, ts.nev);
+    // A rolling window over the events: event e + 1 is read once, and the
+    // neighbour fields of event e - 1 are kept from the previous step
+    // (node_values on registers).
+    const int nev = ts.nev;
+    Event x{}, y{};
+    if (nev > 0) x = ev[0];
+    int prev_node = -2, prev_kind = kNone;
+    double prev_v = 0.;
+    for (int e = 0; e < nev && nb < K; ++e, x = y) {
+      const bool has_next = e + 1 < nev;
+      if (has_next) y = ev[e + 1];
+      NodeValues v;
+      v.lu = prev_node == x.node - 1 && prev_kind == kUncertain;
+      v.ru = has_next && y.node == x.node + 1 && y.kind == kUncertain;
+      v.l = v.lu ? prev_v : double(x.fl);
+      v.r = v.ru ? y.v : double(x.fr);
+      v.c = x.kind == kUncertain ? x.v : double(x.fc);
+      prev_node = x.node;
+      prev_kind = x.kind;
+      prev_v = x.v;
       const bool split = x.probe && x.x == x.x, split_left = split && x.x < x.c1;
       // Left cell [c0, c1]: an uncertain node owns it when that neighbour is certain.
       if (split_left) emit_pair(x, x.node - 1, x.c0, x.c1, v.l, v.c);
