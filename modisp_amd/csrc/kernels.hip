@@ -720,13 +720,25 @@ struct NodeValues {
   double l, c, r;
   bool lu, ru;
 };
-__device__ inline NodeValues node_values(const Event *ev, int e, int nev) {
-  const Event &x = ev[e];
+// The fields of an event that its neighbours' values depend on.
+struct NodeFields {
+  int node, kind;
+  double v;
+};
+__device__ inline NodeFields node_fields(const Event &x) { return NodeFields{x.node, x.kind, x.v}; }
+// Node fields of the slots beside event e, clamped into the trace's kEventCap
+// slots so that they can be read together with the event itself (a slot at
+// or past the trace's event count is ignored by node_values).
+__device__ inline void neighbour_fields(const Event *ev, int e, NodeFields &l, NodeFields &r) {
+  l = node_fields(ev[e > 0 ? e - 1 : e]);
+  r = node_fields(ev[e + 1 < kEventCap ? e + 1 : e]);
+}
+__device__ inline NodeValues node_values(const Event &x, int e, int nev, const NodeFields &l, const NodeFields &r) {
   NodeValues n;
-  n.lu = e > 0 && ev[e - 1].node == x.node - 1 && ev[e - 1].kind == kUncertain;
-  n.ru = e + 1 < nev && ev[e + 1].node == x.node + 1 && ev[e + 1].kind == kUncertain;
-  n.l = n.lu ? ev[e - 1].v : double(x.fl);
-  n.r = n.ru ? ev[e + 1].v : double(x.fr);
+  n.lu = e > 0 && l.node == x.node - 1 && l.kind == kUncertain;
+  n.ru = e + 1 < nev && r.node == x.node + 1 && r.kind == kUncertain;
+  n.l = n.lu ? l.v : double(x.fl);
+  n.r = n.ru ? r.v : double(x.fr);
   n.c = x.kind == kUncertain ? x.v : double(x.fc);
   return n;
 }
@@ -748,14 +760,22 @@ __global__ __launch_bounds__(kBlock) void unc_probe_kernel(PipeArgs a) {
       g = a.unc_queue[q];
       const int tl = g / kEventCap, e = g % kEventCap;
       const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
-      const Event &x = ev[e];
+      // Every read this lane needs is issued at once: the event, the node
+      // fields of the slots beside it (in bounds of the trace's kEventCap
+      // slots; a slot at or past nev is ignored below), the trace's scan
+      // record and the model's FP32 limit.
+      const Event x = ev[e];
+      NodeFields xl, xr;
+      neighbour_fields(ev, e, xl, xr);
+      const unsigned tflags = a.scan[tl].flags;
+      const int nev = a.scan[tl].nev;
       // The FP32 probe pass cannot work below the FP32 limit (a root-free band).
       const double low = a.fp32 ? a.models[a.traces[tl + a.trace_begin].model].stable_low : 0.;
       const int kind = x.probe & 0xff;
       const bool has_l = x.c0 == x.c0, has_r = x.c2 == x.c2;
-      const bool phase_ready = (a.scan[tl].flags & kScanPhaseReady) != 0;  // no pair probes (compact_kernel)
+      const bool phase_ready = (tflags & kScanPhaseReady) != 0;  // no pair probes (compact_kernel)
       if (!phase_ready && kind != kTrough && (has_l || has_r) && !(has_l && x.c0 < low) && !(!has_l && x.c1 < low)) {
-        const NodeValues v = node_values(ev, e, a.scan[tl].nev);
+        const NodeValues v = node_values(x, e, nev, xl, xr);
         // Both neighbours, or the one neighbour of a window end.
         const double al = has_l ? fabs(v.l) : INFINITY, ar = has_r ? fabs(v.r) : INFINITY;
         const bool falls = (!has_l || signbit(v.l) == signbit(v.c)) && (!has_r || signbit(v.r) == signbit(v.c)) &&
@@ -924,18 +944,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         const int q = (wave_id + (v >> 6) * nwaves) * 64 + (v & 63);
         if (v < stream && q < n) {
           g = from_list ? a.handoff_list[q] : a.trough_queue[q];
-          const int tl = g / kEventCap;
-          const Event &x = a.events[g];
+          const int tl = g / kEventCap, e = g % kEventCap;
+          // The event, the node fields of the slots beside it (in bounds of
+          // the trace's slots; one at or past nev is ignored), the trace's
+          // event count and its TraceInfo are read together.
+          const Event *ev = a.events + static_cast<size_t>(tl) * kEventCap;
+          const Event x = ev[e];
+          NodeFields xl, xr;
+          neighbour_fields(ev, e, xl, xr);
+          const int nev = a.scan[tl].nev;
+          const TraceInfo ti = a.traces[tl + a.trace_begin];
           const int kind = x.probe & 0xff;
           const bool resumed = !fp32 && a.fp32;
           if ((kind == kTrough || kind == kDip) && !(resumed && !(x.probe & kProbeNeeds64))) {
             // Search only the neighbouring cells whose end values share the node's sign.
-            const NodeValues nv = node_values(a.events + static_cast<size_t>(tl) * kEventCap, g % kEventCap,
-                                              a.scan[tl].nev);
+            const NodeValues nv = node_values(x, e, nev, xl, xr);
             const bool left = x.c0 == x.c0 && signbit(nv.l) == signbit(nv.c);
             const bool right = x.c2 == x.c2 && signbit(nv.r) == signbit(nv.c);
             if (left || right) {
-              const TraceInfo ti = a.traces[tl + a.trace_begin];
               const ModelInfo mi = a.models[ti.model];
               rows = a.rows + mi.row0;
               L = mi.L;
@@ -1037,26 +1063,16 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(PipeArgs a) {
       emit(lo, x.x, flo, x.vx);
       if (nb < K) emit(x.x, hi, x.vx, fhi);
     };
-    // A rolling window over the events: event e + 1 is read once, and the
-    // neighbour fields of event e - 1 are kept from the previous step
-    // (node_values on registers).
+    // A rolling window over the events: each event is read once, and the
+    // node fields of event e - 1 are kept from the previous step.
     const int nev = ts.nev;
     Event x{}, y{};
     if (nev > 0) x = ev[0];
-    int prev_node = -2, prev_kind = kNone;
-    double prev_v = 0.;
+    NodeFields prev{-2, kNone, 0.};
     for (int e = 0; e < nev && nb < K; ++e, x = y) {
-      const bool has_next = e + 1 < nev;
-      if (has_next) y = ev[e + 1];
-      NodeValues v;
-      v.lu = prev_node == x.node - 1 && prev_kind == kUncertain;
-      v.ru = has_next && y.node == x.node + 1 && y.kind == kUncertain;
-      v.l = v.lu ? prev_v : double(x.fl);
-      v.r = v.ru ? y.v : double(x.fr);
-      v.c = x.kind == kUncertain ? x.v : double(x.fc);
-      prev_node = x.node;
-      prev_kind = x.kind;
-      prev_v = x.v;
+      if (e + 1 < nev) y = ev[e + 1];
+      const NodeValues v = node_values(x, e, nev, prev, node_fields(y));
+      prev = node_fields(x);
       const bool split = x.probe && x.x == x.x, split_left = split && x.x < x.c1;
       // Left cell [c0, c1]: an uncertain node owns it when that neighbour is certain.
       if (split_left) emit_pair(x, x.node - 1, x.c0, x.c1, v.l, v.c);
