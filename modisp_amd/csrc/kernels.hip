@@ -894,9 +894,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   const int n = min(from_list ? *a.handoff_count : *a.trough_count, a.queue_cap);
   const int lane = threadIdx.x & 63;
   const int wave_id = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  const int nwaves = gridDim.x * (kBlock / 64);
+  // Runs of 64 entries, dealt round-robin over the waves: at least four runs
+  // per wave where the list is long enough to keep every resident wave busy
+  // (a wave that holds a longer stream refills its lanes instead of ending
+  // each run with a tail), and one wave per run below that.
   const int nruns = (n + 63) / 64;
-  const int stream = wave_id < nruns ? ((nruns - 1 - wave_id) / nwaves + 1) * 64 : 0;
+  const int nwaves =
+      min(static_cast<int>(gridDim.x) * (kBlock / 64), max(a.probe_waves[fp32 ? 0 : 1], (nruns + 3) / 4));
+  const int stream = wave_id < nruns && wave_id < nwaves ? ((nruns - 1 - wave_id) / nwaves + 1) * 64 : 0;
   int cursor = 0;  // wave-uniform: next entry of the wave's stream
   GoldenState s{};
   bool active = false;
@@ -2168,6 +2173,19 @@ static hipError_t launch_scan_t(const PipeArgs &a, hipStream_t s) {
     launch_scan_w<T, 4>(a, n, s);
   }
   return hipGetLastError();
+}
+
+hipError_t probe_residency(int cus, int wave, int out[2]) {
+  int b32 = 0, b64 = 0;
+  hipError_t e = wave == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b32, trough_kernel<float, 1>, kBlock, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b32, trough_kernel<float, 0>, kBlock, 0);
+  if (e != hipSuccess) return e;
+  e = wave == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b64, trough_kernel<double, 1>, kBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b64, trough_kernel<double, 0>, kBlock, 0);
+  if (e != hipSuccess) return e;
+  out[0] = b32 * cus * (kBlock / 64);
+  out[1] = b64 * cus * (kBlock / 64);
+  return hipSuccess;
 }
 
 template <class K>

@@ -170,6 +170,8 @@ struct PipeArgs {
   int close_grid;  // 1: close roots of a scan-accepted trace raise kFlagCloseRoot (0 for high qe_adaptive
                    // Rayleigh, whose search never sets it; Love phase traces always test)
   int love_phase;  // 1: Love roots are verified by the exact phase count (reference love_phase=1)
+  int probe_waves[2];  // probe-kernel waves resident on the device at once (FP32 pass, FP64 pass);
+                      // a probe pass spreads its runs over no fewer waves than this
   unsigned trace_extra;  // flags every searched trace carries (kFlagApproximate for approximate presets)
   double f_accept;  // reported roots with |F| > f_accept are rejected (reference solver.cpp:333); 1 = off
   double tol;
@@ -246,6 +248,9 @@ hipError_t launch_tables(const ModelInfo *models, const PrepRow *rows, int nmode
                          TablePoint *table, int *ntab, hipStream_t s);
 hipError_t launch_pipeline(const ScanArgs &sa, const PipeArgs &pa, bool fp32, bool group, int persistent_blocks,
                            hipStream_t s, hipEvent_t *marks);
+// Waves of trough_kernel<float, wave> (out[0]) and <double, wave> (out[1])
+// resident at once on `cus` compute units.
+hipError_t probe_residency(int cus, int wave, int out[2]);
 
 }  // namespace mdr
 

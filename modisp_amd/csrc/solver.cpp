@@ -233,6 +233,7 @@ struct md_rocm_solver {
   DevBuf invalid;                  // models of the last device ragged run with inconsistent offsets
   HostStage host_stage;            // pinned staging of host-array results
   bool invalid_live = false;       // `invalid` belongs to the last run
+  int probe_waves[2][2] = {};      // [wave type] resident probe waves (FP32, FP64 pass); 0 if the query failed
 
   ~md_rocm_solver() {
     for (DevBuf *b : {&rows, &models, &traces, &events, &scan, &brackets, &roots, &unc_q, &tr_q, &counters, &stats,
@@ -397,6 +398,8 @@ int run_pipeline(md_rocm_solver *s, const Work &w, long long ntraces, int kmax, 
   a.level_grid = grid_level(s->cfg);
   a.close_grid = s->cfg.shape || s->cfg.accuracy != 0 ? 1 : 0;
   a.love_phase = s->cfg.love_phase;
+  a.probe_waves[0] = s->probe_waves[wave][0];
+  a.probe_waves[1] = s->probe_waves[wave][1];
   a.trace_extra = s->cfg.accuracy != 0 ? mdr::kFlagApproximate : 0u;
   a.f_accept = s->f_accept;
   a.tol = tol;
@@ -757,6 +760,11 @@ md_rocm_solver *md_rocm_create(int capacity, int L, int nf, int nmodes, const md
   scratch_setup(s->device);
   hipDeviceProp_t p;
   if (hipGetDeviceProperties(&p, s->device) == hipSuccess) s->cus = p.multiProcessorCount;
+  // Resident probe waves per wave type (an occupancy query, made here so that
+  // no run, captured or not, has to make it).
+  for (int w = 0; w < 2; ++w)
+    if (mdr::probe_residency(s->cus, w, s->probe_waves[w]) != hipSuccess)
+      s->probe_waves[w][0] = s->probe_waves[w][1] = 0;
   if (hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&s->ev_begin) != hipSuccess || hipEventCreate(&s->ev_end) != hipSuccess) {
     delete s;
