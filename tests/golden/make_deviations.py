@@ -26,7 +26,7 @@ anything else stops the script:
   shape search, which returned the same roots; the device's phase count
   accepted them (level 11, no 16384 flag).
 
-    python tests/golden/make_deviations.py gpurun_out/deviations_raw.json
+    python tests/golden/make_deviations.py <dump.json>
 """
 from __future__ import annotations
 
